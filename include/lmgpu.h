@@ -1,0 +1,220 @@
+/*
+ * lmgpu.h -- C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * Plain C types only: pointers, sizes, status codes.  No exception crosses this
+ * boundary; every entry point returns an lmgpu_status and lmgpu_last_error() gives
+ * the message of the last failure on a context.
+ *
+ * Each entry point replaces one piece of the reference's Scene / Renderer /
+ * RenderProcessScheduler / Film path (paths relative to the reference root):
+ *
+ *   lmgpu_scene_upload   Scene::Load + Primitives flattening
+ *                        (include/lightmetrica/scene.h:66-69, src/liblightmetrica/primitives.cpp:251-418),
+ *                        AreaLight::RegisterPrimitives (src/liblightmetrica/light.area.cpp:109-144),
+ *                        PerspectiveCamera::Load/RegisterPrimitives (src/liblightmetrica/perspectivecamera.cpp:97-184),
+ *                        ConstantEnvironmentLight::PostConfigure (src/liblightmetrica/light.env.const.cpp:88-98)
+ *   lmgpu_bvh_build      Scene::Build (include/lightmetrica/scene.h:144); QBVHScene::Build
+ *                        (src/liblightmetrica/scene.qbvh.cpp:453-521) / BVHScene::Build (scene.bvh.cpp:202-388)
+ *   lmgpu_trace_closest  Scene::IntersectTriangles (include/lightmetrica/scene.h:156);
+ *                        QBVHScene::IntersectTriangles (src/liblightmetrica/scene.qbvh.cpp:837-960)
+ *   lmgpu_trace_any      RenderUtils::Visible (src/liblightmetrica/renderutils.cpp:58-70), which the
+ *                        reference answers with a full closest-hit query
+ *   lmgpu_pt_render      MTRenderProcessScheduler::Render (src/liblightmetrica/sched.mt.cpp:93-266) driving
+ *                        PathtraceRenderer / DirectPathtraceRenderer / MISPathtraceRenderer::ProcessSingleSample
+ *                        (src/liblightmetrica/pathtrace.cpp:165-266, pathtrace.direct.cpp:170-338,
+ *                        pathtrace.mis.cpp:167-369) and HDRBitmapFilm::AccumulateContribution + Rescale
+ *                        (src/liblightmetrica/hdrfilm.cpp:151-171, :334-341)
+ *
+ * Threading: a context is driven by one host thread.  All work of a context runs
+ * on the context's HIP stream (its own, or the one set with lmgpu_set_stream).
+ * Buffers marked "device" are device pointers on the context's device; all other
+ * pointers are host memory owned by the caller.
+ */
+#ifndef LMGPU_H
+#define LMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMGPU_ABI_VERSION 1
+
+typedef int lmgpu_status;
+#define LMGPU_OK 0
+#define LMGPU_ERR_INVALID_ARGUMENT 1
+#define LMGPU_ERR_HIP 2
+#define LMGPU_ERR_NO_SCENE 3
+#define LMGPU_ERR_NO_BVH 4
+#define LMGPU_ERR_UNSUPPORTED 5
+#define LMGPU_ERR_OUT_OF_MEMORY 6
+#define LMGPU_ERR_BVH_TOO_DEEP 7
+#define LMGPU_ERR_INJECTED 8
+
+typedef struct lmgpu_ctx lmgpu_ctx;
+
+/* Materials -- DiffuseBSDF (src/liblightmetrica/diffuse.cpp), PerfectMirrorBSDF (mirror.cpp),
+ * DiffuseMirrorMixBSDF (plugin/plugin.diffusemirror/diffusemirror.cpp). */
+#define LMGPU_BSDF_DIFFUSE 0u
+#define LMGPU_BSDF_MIRROR 1u
+#define LMGPU_BSDF_DIFFUSEMIRROR 2u
+typedef struct lmgpu_material {
+    uint32_t type;
+    float R[3]; /* diffuse / specular reflectance */
+} lmgpu_material;
+
+/* Lights -- AreaLight (src/liblightmetrica/light.area.cpp), ConstantEnvironmentLight
+ * (light.env.const.cpp).  The order is the light-selection order of
+ * Scene::SampleLightSelection (src/liblightmetrica/scene.cpp:79-86); the reference
+ * puts the environment light first (src/liblightmetrica/primitives.cpp:140). */
+#define LMGPU_LIGHT_AREA 0u
+#define LMGPU_LIGHT_ENV_CONST 1u
+typedef struct lmgpu_light {
+    uint32_t type;
+    float Le[3]; /* luminance */
+} lmgpu_light;
+
+/* Pinhole camera: view = world-to-camera 4x4, column-major (the camera primitive's
+ * transform, perspectivecamera.cpp:174-184); projection = Perspective(fovy, W/H, 1, 1000). */
+typedef struct lmgpu_camera {
+    float view[16];
+    float fovy_deg;
+    uint32_t width, height;
+} lmgpu_camera;
+
+/* World-space triangle soup (primitive transforms already applied).  Normals are
+ * per vertex and required, as in Scene::StoreIntersectionFromBarycentricCoords
+ * (src/liblightmetrica/scene.cpp:101-149). */
+typedef struct lmgpu_scene_desc {
+    const float* positions;      /* num_vertices * 3 */
+    const float* normals;        /* num_vertices * 3 */
+    uint32_t num_vertices;
+    const uint32_t* faces;       /* num_triangles * 3 */
+    uint32_t num_triangles;
+    const uint32_t* material_ids; /* num_triangles, index into materials (may be NULL for trace-only scenes) */
+    const int32_t* light_ids;    /* num_triangles, index into lights or -1 (may be NULL) */
+    const lmgpu_material* materials;
+    uint32_t num_materials;
+    const lmgpu_light* lights;
+    uint32_t num_lights;
+    const lmgpu_camera* camera;  /* may be NULL for trace-only scenes */
+} lmgpu_scene_desc;
+
+/* Wide-BVH build: binned SAH on the host (same SAH family as the reference builders),
+ * collapsed to a 4-wide tree and uploaded to HBM. */
+typedef struct lmgpu_bvh_params {
+    uint32_t max_leaf_size;   /* triangles per leaf, 1..16 (0 = default 4) */
+    uint32_t sah_bins;        /* 0 = default 16 */
+    float traversal_cost;     /* relative to one triangle test; <= 0 = default 1.2 */
+    int num_threads;          /* host build threads, <= 0 = all */
+} lmgpu_bvh_params;
+
+typedef struct lmgpu_bvh_info {
+    uint32_t num_nodes;       /* 4-wide nodes (128 B each) */
+    uint32_t num_leaves;
+    uint32_t max_depth;       /* inner-node levels on the deepest path */
+    uint32_t num_tri_records; /* 48 B each */
+    double build_seconds;
+    double sah_cost;
+    uint64_t device_bytes;
+} lmgpu_bvh_info;
+
+/* Rays and hits, structure of arrays.  device != 0: the pointers are device memory. */
+typedef struct lmgpu_rays_soa {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+    int device;
+} lmgpu_rays_soa;
+
+typedef struct lmgpu_hits_soa {
+    float *t, *b0, *b1; /* b0, b1 = weights of triangle vertices 2 and 3 (scene.cpp:133) */
+    int32_t* prim;      /* triangle index in scene order, -1 = miss */
+    int device;
+} lmgpu_hits_soa;
+
+/* lmgpu_trace_* flags */
+#define LMGPU_TRACE_REFINE_FP64 1u /* re-evaluate the winning triangle in FP64 (t, b0, b1) */
+
+/* Path tracer */
+#define LMGPU_PT 0        /* renderer "pt"        -- pathtrace.cpp */
+#define LMGPU_PT_DIRECT 1 /* renderer "pt.direct" -- pathtrace.direct.cpp */
+#define LMGPU_PT_MIS 2    /* renderer "pt.mis"    -- pathtrace.mis.cpp */
+
+#define LMGPU_FILM_FP32 0
+#define LMGPU_FILM_FP64 1
+#define LMGPU_FILM_FP32_COMPENSATED 2
+
+typedef struct lmgpu_pt_params {
+    int renderer;             /* LMGPU_PT, LMGPU_PT_DIRECT, LMGPU_PT_MIS */
+    uint64_t num_samples;     /* path samples (the scheduler's num_samples) */
+    int max_path_vertices;    /* -1 = unbounded (pathtrace.cpp:126) */
+    int rr_depth;             /* -1 = no Russian roulette (pathtrace.cpp:125) */
+    uint64_t seed;            /* per-sample Philox4x32-10 stream key */
+    uint64_t sample_offset;   /* global index of the first sample (shards of a larger render) */
+    uint32_t wavefront_width; /* paths in flight per batch; 0 = auto */
+    int film_precision;       /* LMGPU_FILM_* accumulation type on the device */
+} lmgpu_pt_params;
+
+/* Output film: width*height*3 values, rows bottom-up (raster y = 0 is the bottom row),
+ * RGB interleaved, already rescaled by W*H/num_samples (sched.mt.cpp:256).
+ * precision: LMGPU_FILM_FP32 -> float data, LMGPU_FILM_FP64 -> double data. */
+typedef struct lmgpu_film {
+    void* data;
+    int device;
+    int precision;
+    uint32_t width, height;
+} lmgpu_film;
+
+typedef struct lmgpu_pt_stats {
+    uint64_t closest_rays; /* extension rays traced */
+    uint64_t shadow_rays;  /* any-hit rays traced */
+    uint64_t samples;
+    double seconds;        /* device time of the render (HIP events) */
+} lmgpu_pt_stats;
+
+/* ---- context ---------------------------------------------------------------- */
+lmgpu_status lmgpu_create(int device, lmgpu_ctx** out);
+lmgpu_status lmgpu_destroy(lmgpu_ctx* ctx);
+const char* lmgpu_last_error(const lmgpu_ctx* ctx);
+int lmgpu_abi_version(void);
+/* Use an external HIP stream (hipStream_t passed as void*); NULL restores the context's own. */
+lmgpu_status lmgpu_set_stream(lmgpu_ctx* ctx, void* hip_stream);
+lmgpu_status lmgpu_synchronize(lmgpu_ctx* ctx);
+
+/* ---- scene / accelerator ------------------------------------------------------ */
+lmgpu_status lmgpu_scene_upload(lmgpu_ctx* ctx, const lmgpu_scene_desc* desc);
+lmgpu_status lmgpu_bvh_build(lmgpu_ctx* ctx, const lmgpu_bvh_params* params, lmgpu_bvh_info* info);
+
+/* ---- traversal ---------------------------------------------------------------- */
+lmgpu_status lmgpu_trace_closest(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint32_t n, lmgpu_hits_soa* hits,
+                                 uint32_t flags);
+lmgpu_status lmgpu_trace_any(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint32_t n, uint8_t* occluded,
+                             int occluded_on_device, uint32_t flags);
+/* Counting run of the closest-hit traversal: total 4-wide node visits and triangle tests. */
+lmgpu_status lmgpu_trace_count(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint32_t n, uint64_t* node_visits,
+                               uint64_t* tri_tests);
+
+/* ---- path tracing ------------------------------------------------------------- */
+lmgpu_status lmgpu_pt_render(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_film* film, lmgpu_pt_stats* stats);
+
+/* ---- timing ------------------------------------------------------------------- */
+/* When enabled, every traversal / path-tracing kernel launch is bracketed by HIP
+ * events on the launch stream; lmgpu_kernel_timing returns the summed milliseconds
+ * and launch count for one kernel class since the last reset.
+ * kernel: 0 = closest-hit traversal, 1 = any-hit traversal, 2 = pt shading stages,
+ *         3 = pt generation, 4 = pt shadow (any-hit) stage, 5 = pt extension (closest-hit) stage */
+lmgpu_status lmgpu_enable_timing(lmgpu_ctx* ctx, int enable);
+lmgpu_status lmgpu_kernel_timing(lmgpu_ctx* ctx, int kernel, double* total_ms, uint64_t* launches);
+lmgpu_status lmgpu_reset_timing(lmgpu_ctx* ctx);
+
+/* ---- test hooks --------------------------------------------------------------- */
+/* Makes the next n calls that allocate device memory fail with LMGPU_ERR_INJECTED
+ * (a simulated allocation failure for the host-side error-path tests). */
+lmgpu_status lmgpu_inject_alloc_failures(lmgpu_ctx* ctx, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LMGPU_H */

@@ -1,0 +1,132 @@
+"""ctypes declarations of include/lmgpu.h and the loader of liblmgpu.so.
+
+The library is built in-tree (``python -m lightmetrica_amd.build``).  There is no
+fallback: if the library is missing or cannot be loaded, importing the GPU API raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liblmgpu.so"
+
+LMGPU_OK = 0
+STATUS_NAMES = {
+    0: "LMGPU_OK", 1: "LMGPU_ERR_INVALID_ARGUMENT", 2: "LMGPU_ERR_HIP", 3: "LMGPU_ERR_NO_SCENE",
+    4: "LMGPU_ERR_NO_BVH", 5: "LMGPU_ERR_UNSUPPORTED", 6: "LMGPU_ERR_OUT_OF_MEMORY", 7: "LMGPU_ERR_BVH_TOO_DEEP",
+    8: "LMGPU_ERR_INJECTED",
+}
+
+LMGPU_TRACE_REFINE_FP64 = 1
+LMGPU_PT, LMGPU_PT_DIRECT, LMGPU_PT_MIS = 0, 1, 2
+LMGPU_FILM_FP32, LMGPU_FILM_FP64, LMGPU_FILM_FP32_COMPENSATED = 0, 1, 2
+
+# Every exported symbol of include/lmgpu.h.
+EXPORTS = [
+    "lmgpu_create", "lmgpu_destroy", "lmgpu_last_error", "lmgpu_abi_version", "lmgpu_set_stream",
+    "lmgpu_synchronize", "lmgpu_scene_upload", "lmgpu_bvh_build", "lmgpu_trace_closest", "lmgpu_trace_any",
+    "lmgpu_trace_count", "lmgpu_pt_render", "lmgpu_enable_timing", "lmgpu_kernel_timing", "lmgpu_reset_timing",
+    "lmgpu_inject_alloc_failures",
+]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("R", C.c_float * 3)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("Le", C.c_float * 3)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("fovy_deg", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("positions", C.POINTER(C.c_float)),
+        ("normals", C.POINTER(C.c_float)),
+        ("num_vertices", C.c_uint32),
+        ("faces", C.POINTER(C.c_uint32)),
+        ("num_triangles", C.c_uint32),
+        ("material_ids", C.POINTER(C.c_uint32)),
+        ("light_ids", C.POINTER(C.c_int32)),
+        ("materials", C.POINTER(Material)),
+        ("num_materials", C.c_uint32),
+        ("lights", C.POINTER(Light)),
+        ("num_lights", C.c_uint32),
+        ("camera", C.POINTER(CameraDesc)),
+    ]
+
+
+class BvhParams(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_uint32), ("sah_bins", C.c_uint32), ("traversal_cost", C.c_float),
+                ("num_threads", C.c_int)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint32), ("num_leaves", C.c_uint32), ("max_depth", C.c_uint32),
+                ("num_tri_records", C.c_uint32), ("build_seconds", C.c_double), ("sah_cost", C.c_double),
+                ("device_bytes", C.c_uint64)]
+
+
+class RaysSoA(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")] + [("device", C.c_int)]
+
+
+class HitsSoA(C.Structure):
+    _fields_ = [("t", C.c_void_p), ("b0", C.c_void_p), ("b1", C.c_void_p), ("prim", C.c_void_p), ("device", C.c_int)]
+
+
+class PtParams(C.Structure):
+    _fields_ = [("renderer", C.c_int), ("num_samples", C.c_uint64), ("max_path_vertices", C.c_int),
+                ("rr_depth", C.c_int), ("seed", C.c_uint64), ("sample_offset", C.c_uint64),
+                ("wavefront_width", C.c_uint32), ("film_precision", C.c_int)]
+
+
+class Film(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("device", C.c_int), ("precision", C.c_int), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
+
+
+class PtStats(C.Structure):
+    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
+                ("seconds", C.c_double)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load liblmgpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m lightmetrica_amd.build` first")
+    lib = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    st = C.c_int
+    lib.lmgpu_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.lmgpu_destroy.argtypes = [vp]
+    lib.lmgpu_last_error.argtypes = [vp]
+    lib.lmgpu_last_error.restype = C.c_char_p
+    lib.lmgpu_abi_version.argtypes = []
+    lib.lmgpu_set_stream.argtypes = [vp, vp]
+    lib.lmgpu_synchronize.argtypes = [vp]
+    lib.lmgpu_scene_upload.argtypes = [vp, C.POINTER(SceneDesc)]
+    lib.lmgpu_bvh_build.argtypes = [vp, C.POINTER(BvhParams), C.POINTER(BvhInfo)]
+    lib.lmgpu_trace_closest.argtypes = [vp, C.POINTER(RaysSoA), C.c_uint32, C.POINTER(HitsSoA), C.c_uint32]
+    lib.lmgpu_trace_any.argtypes = [vp, C.POINTER(RaysSoA), C.c_uint32, vp, C.c_int, C.c_uint32]
+    lib.lmgpu_trace_count.argtypes = [vp, C.POINTER(RaysSoA), C.c_uint32, C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]
+    lib.lmgpu_pt_render.argtypes = [vp, C.POINTER(PtParams), C.POINTER(Film), C.POINTER(PtStats)]
+    lib.lmgpu_enable_timing.argtypes = [vp, C.c_int]
+    lib.lmgpu_kernel_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    lib.lmgpu_reset_timing.argtypes = [vp]
+    lib.lmgpu_inject_alloc_failures.argtypes = [vp, C.c_int]
+    for name in EXPORTS:
+        if name not in ("lmgpu_last_error",):
+            getattr(lib, name).restype = st
+    _lib = lib
+    return lib

@@ -1,0 +1,361 @@
+// Host-side wide-BVH builder (see bvh_build.h).
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <cmath>
+#include <limits>
+#include <thread>
+
+namespace lmgpu {
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+    void reset() {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::numeric_limits<float>::infinity();
+            hi[a] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const Box& b) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], b.lo[a]);
+            hi[a] = std::max(hi[a], b.hi[a]);
+        }
+    }
+    void grow(const float* p) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    bool empty() const { return lo[0] > hi[0]; }
+    float area() const {
+        if (empty()) return 0.f;
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return 2.f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Prim {
+    Box box;
+    float c[3];
+    uint32_t tri;
+};
+
+struct BinNode {
+    Box box;
+    int32_t left = -1, right = -1;
+    uint32_t first = 0, count = 0;  // leaf range in prims
+    bool leaf() const { return left < 0; }
+};
+
+class Builder {
+  public:
+    Builder(std::vector<Prim>& prims, const BuildParams& p, int threads) : prims_(prims), p_(p) {
+        nodes_.resize(2 * prims.size() + 1);
+        while ((1 << spawn_depth_) < threads) spawn_depth_++;
+    }
+
+    uint32_t build_root() {
+        uint32_t root = alloc();
+        build(root, 0, uint32_t(prims_.size()), 0);
+        nodes_.resize(next_.load());
+        return root;
+    }
+
+    std::vector<BinNode>& nodes() { return nodes_; }
+
+  private:
+    uint32_t alloc() { return next_.fetch_add(1); }
+
+    void make_leaf(uint32_t ni, uint32_t begin, uint32_t end) {
+        nodes_[ni].first = begin;
+        nodes_[ni].count = end - begin;
+        nodes_[ni].left = nodes_[ni].right = -1;
+    }
+
+    void build(uint32_t ni, uint32_t begin, uint32_t end, int depth) {
+        BinNode& nd = nodes_[ni];
+        nd.box.reset();
+        Box cb;
+        cb.reset();
+        for (uint32_t i = begin; i < end; i++) {
+            nd.box.grow(prims_[i].box);
+            cb.grow(prims_[i].c);
+        }
+        const uint32_t n = end - begin;
+        if (n <= 1) { make_leaf(ni, begin, end); return; }
+
+        // Binned SAH over all three axes.
+        const uint32_t B = p_.bins;
+        float best_cost = std::numeric_limits<float>::infinity();
+        int best_axis = -1;
+        uint32_t best_bin = 0;
+        const float parent_area = std::max(nd.box.area(), 1e-30f);
+        if (depth < 64) {
+            std::vector<Box> bb(B);
+            std::vector<uint32_t> bc(B);
+            std::vector<float> right_area(B);
+            std::vector<uint32_t> right_cnt(B);
+            for (int a = 0; a < 3; a++) {
+                const float ext = cb.hi[a] - cb.lo[a];
+                if (!(ext > 0.f)) continue;
+                const float k = float(B) / ext;
+                for (uint32_t b = 0; b < B; b++) { bb[b].reset(); bc[b] = 0; }
+                for (uint32_t i = begin; i < end; i++) {
+                    int b = int((prims_[i].c[a] - cb.lo[a]) * k);
+                    b = std::min(std::max(b, 0), int(B) - 1);
+                    bb[size_t(b)].grow(prims_[i].box);
+                    bc[size_t(b)]++;
+                }
+                Box acc;
+                acc.reset();
+                uint32_t cnt = 0;
+                for (int b = int(B) - 1; b >= 1; b--) {
+                    acc.grow(bb[size_t(b)]);
+                    cnt += bc[size_t(b)];
+                    right_area[size_t(b)] = acc.area();
+                    right_cnt[size_t(b)] = cnt;
+                }
+                acc.reset();
+                cnt = 0;
+                for (uint32_t b = 0; b + 1 < B; b++) {
+                    acc.grow(bb[b]);
+                    cnt += bc[b];
+                    const uint32_t rc = right_cnt[b + 1];
+                    if (cnt == 0 || rc == 0) continue;
+                    const float cost = p_.traversal_cost +
+                                       (float(cnt) * acc.area() + float(rc) * right_area[b + 1]) / parent_area;
+                    if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = b; }
+                }
+            }
+        }
+
+        const float leaf_cost = float(n);
+        if (n <= p_.max_leaf_size && !(best_cost < leaf_cost)) { make_leaf(ni, begin, end); return; }
+
+        uint32_t mid = begin;
+        if (best_axis >= 0) {
+            const int a = best_axis;
+            const float k = float(B) / (cb.hi[a] - cb.lo[a]);
+            Prim* first = prims_.data() + begin;
+            Prim* last = prims_.data() + end;
+            Prim* m = std::partition(first, last, [&](const Prim& p) {
+                int b = int((p.c[a] - cb.lo[a]) * k);
+                b = std::min(std::max(b, 0), int(B) - 1);
+                return uint32_t(b) <= best_bin;
+            });
+            mid = uint32_t(m - prims_.data());
+        }
+        if (mid == begin || mid == end) {
+            // No usable SAH split (identical centroids, or too deep): object median on the longest axis.
+            if (n <= p_.max_leaf_size) { make_leaf(ni, begin, end); return; }
+            int a = 0;
+            for (int k = 1; k < 3; k++)
+                if (cb.hi[k] - cb.lo[k] > cb.hi[a] - cb.lo[a]) a = k;
+            mid = begin + n / 2;
+            std::nth_element(prims_.begin() + begin, prims_.begin() + mid, prims_.begin() + end,
+                             [a](const Prim& x, const Prim& y) { return x.c[a] < y.c[a]; });
+        }
+
+        uint32_t l = alloc(), r = alloc();
+        nodes_[ni].left = int32_t(l);
+        nodes_[ni].right = int32_t(r);
+        if (n > 65536 && depth < spawn_depth_) {
+            // Top levels: one side on a new thread, the other on this one.
+            std::thread side([this, l, begin, mid, depth]() { build(l, begin, mid, depth + 1); });
+            build(r, mid, end, depth + 1);
+            side.join();
+        } else {
+            build(l, begin, mid, depth + 1);
+            build(r, mid, end, depth + 1);
+        }
+    }
+
+    std::vector<Prim>& prims_;
+    const BuildParams& p_;
+    std::vector<BinNode> nodes_;
+    std::atomic<uint32_t> next_{0};
+    int spawn_depth_ = 0;
+};
+
+class Collapser {
+  public:
+    Collapser(const std::vector<BinNode>& bn, const std::vector<Prim>& prims, const float* pos, const uint32_t* faces,
+              WideBVH& out)
+        : bn_(bn), prims_(prims), pos_(pos), faces_(faces), out_(out) {}
+
+    bool run(uint32_t root, std::string& err) {
+        out_.nodes.clear();
+        out_.tris.clear();
+        out_.tris.reserve(prims_.size());
+        out_.num_leaves = 0;
+        out_.max_depth = 0;
+        if (bn_[root].leaf()) {
+            // A single leaf: wrap it in one node so traversal always starts at an inner node.
+            out_.nodes.emplace_back();
+            init_node(out_.nodes.back());
+            set_child(0, 0, root, err);
+            out_.max_depth = 1;
+            return err.empty();
+        }
+        emit(root, 1, err);
+        return err.empty();
+    }
+
+  private:
+    static void init_node(Node4& n) {
+        for (int i = 0; i < 4; i++) {
+            n.lo_x[i] = n.lo_y[i] = n.lo_z[i] = std::numeric_limits<float>::infinity();
+            n.hi_x[i] = n.hi_y[i] = n.hi_z[i] = -std::numeric_limits<float>::infinity();
+            n.child[i] = kEmptyChild;
+            n.pad[i] = 0;
+        }
+    }
+
+    uint32_t emit_leaf(uint32_t b, std::string& err) {
+        const BinNode& nd = bn_[b];
+        uint32_t first = uint32_t(out_.tris.size());
+        if (nd.count == 0 || nd.count > kMaxLeafCount) { err = "leaf size out of range"; return kEmptyChild; }
+        if (first + nd.count - 1 > kMaxLeafFirst) { err = "too many triangles for 27-bit leaf offsets"; return kEmptyChild; }
+        for (uint32_t i = nd.first; i < nd.first + nd.count; i++) {
+            const uint32_t t = prims_[i].tri;
+            TriRecord r;
+            const float* p[3] = {pos_ + 3 * size_t(faces_[3 * size_t(t)]), pos_ + 3 * size_t(faces_[3 * size_t(t) + 1]),
+                                 pos_ + 3 * size_t(faces_[3 * size_t(t) + 2])};
+            float* dst[3] = {r.v0, r.v1, r.v2};
+            for (int k = 0; k < 3; k++) {
+                dst[k][0] = p[k][0]; dst[k][1] = p[k][1]; dst[k][2] = p[k][2]; dst[k][3] = 0.f;
+            }
+            std::memcpy(&r.v0[3], &t, 4);
+            out_.tris.push_back(r);
+        }
+        out_.num_leaves++;
+        return make_leaf_ref(first, nd.count);
+    }
+
+    void set_child(uint32_t node, int slot, uint32_t b, std::string& err) {
+        const Box& bx = bn_[b].box;
+        Node4& n = out_.nodes[node];
+        n.lo_x[slot] = bx.lo[0]; n.hi_x[slot] = bx.hi[0];
+        n.lo_y[slot] = bx.lo[1]; n.hi_y[slot] = bx.hi[1];
+        n.lo_z[slot] = bx.lo[2]; n.hi_z[slot] = bx.hi[2];
+        if (bn_[b].leaf()) out_.nodes[node].child[slot] = emit_leaf(b, err);
+    }
+
+    // Greedy collapse: open the largest-area inner child until 4 children.
+    uint32_t emit(uint32_t b, uint32_t depth, std::string& err) {
+        out_.max_depth = std::max(out_.max_depth, depth);
+        uint32_t kids[4] = {uint32_t(bn_[b].left), uint32_t(bn_[b].right), 0, 0};
+        int nk = 2;
+        while (nk < 4) {
+            int best = -1;
+            float best_area = -1.f;
+            for (int i = 0; i < nk; i++) {
+                if (bn_[kids[i]].leaf()) continue;
+                float a = bn_[kids[i]].box.area();
+                if (a > best_area) { best_area = a; best = i; }
+            }
+            if (best < 0) break;
+            uint32_t open = kids[best];
+            kids[best] = uint32_t(bn_[open].left);
+            kids[nk++] = uint32_t(bn_[open].right);
+        }
+        uint32_t me = uint32_t(out_.nodes.size());
+        out_.nodes.emplace_back();
+        init_node(out_.nodes[me]);
+        for (int i = 0; i < nk; i++) {
+            set_child(me, i, kids[i], err);
+            if (!err.empty()) return me;
+        }
+        for (int i = 0; i < nk; i++) {
+            if (bn_[kids[i]].leaf()) continue;
+            uint32_t c = emit(kids[i], depth + 1, err);
+            if (!err.empty()) return me;
+            out_.nodes[me].child[i] = c;
+        }
+        return me;
+    }
+
+    const std::vector<BinNode>& bn_;
+    const std::vector<Prim>& prims_;
+    const float* pos_;
+    const uint32_t* faces_;
+    WideBVH& out_;
+};
+
+double sah_of(const WideBVH& w, float ct) {
+    // Expected cost: sum over nodes of area(node)/area(root) * (ct + ...): computed from child boxes.
+    if (w.nodes.empty()) return 0.0;
+    const Node4& r = w.nodes[0];
+    double root_area = 0.0;
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = 0; i < 4; i++) {
+            if (r.child[i] == kEmptyChild) continue;
+            lo[0] = std::min(lo[0], r.lo_x[i]); hi[0] = std::max(hi[0], r.hi_x[i]);
+            lo[1] = std::min(lo[1], r.lo_y[i]); hi[1] = std::max(hi[1], r.hi_y[i]);
+            lo[2] = std::min(lo[2], r.lo_z[i]); hi[2] = std::max(hi[2], r.hi_z[i]);
+        }
+        double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        root_area = 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+    if (!(root_area > 0)) return 0.0;
+    double cost = ct;  // the root is always visited
+    for (const Node4& n : w.nodes) {
+        for (int i = 0; i < 4; i++) {
+            if (n.child[i] == kEmptyChild) continue;
+            double dx = n.hi_x[i] - n.lo_x[i], dy = n.hi_y[i] - n.lo_y[i], dz = n.hi_z[i] - n.lo_z[i];
+            double a = 2.0 * (dx * dy + dy * dz + dz * dx) / root_area;
+            if (n.child[i] & kLeafBit) cost += a * double(((n.child[i] >> 27) & 0xfu) + 1);
+            else cost += a * ct;
+        }
+    }
+    return cost;
+}
+
+}  // namespace
+
+bool build_wide_bvh(const float* positions, uint32_t num_vertices, const uint32_t* faces, uint32_t num_triangles,
+                    const BuildParams& params_in, WideBVH& out, std::string& err) {
+    auto t0 = std::chrono::steady_clock::now();
+    BuildParams p = params_in;
+    if (p.max_leaf_size == 0) p.max_leaf_size = 4;
+    if (p.max_leaf_size > kMaxLeafCount) { err = "max_leaf_size must be <= 16"; return false; }
+    if (p.bins < 2) p.bins = 16;
+    if (!(p.traversal_cost > 0.f)) p.traversal_cost = 1.2f;
+    if (num_triangles == 0) { err = "scene has no triangles"; return false; }
+    if (num_triangles - 1 > kMaxLeafFirst) { err = "too many triangles"; return false; }
+    int threads = p.num_threads > 0 ? p.num_threads : int(std::max(1u, std::thread::hardware_concurrency()));
+
+    std::vector<Prim> prims(num_triangles);
+    bool bad_index = false;
+    for (long long t = 0; t < (long long)num_triangles; t++) {
+        Prim& pr = prims[size_t(t)];
+        pr.tri = uint32_t(t);
+        pr.box.reset();
+        for (int k = 0; k < 3; k++) {
+            uint32_t v = faces[3 * size_t(t) + size_t(k)];
+            if (v >= num_vertices) { bad_index = true; v = 0; }
+            pr.box.grow(positions + 3 * size_t(v));
+        }
+        for (int a = 0; a < 3; a++) pr.c[a] = 0.5f * (pr.box.lo[a] + pr.box.hi[a]);
+    }
+    if (bad_index) {
+        err = "face index out of range";
+        return false;
+    }
+    Builder b(prims, p, threads);
+    uint32_t root = b.build_root();
+    Collapser c(b.nodes(), prims, positions, faces, out);
+    if (!c.run(root, err)) return false;
+    out.sah_cost = sah_of(out, p.traversal_cost);
+    out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return true;
+}
+
+}  // namespace lmgpu

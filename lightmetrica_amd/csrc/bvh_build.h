@@ -1,0 +1,66 @@
+// Host-side wide-BVH builder for the GPU traversal kernels.
+//
+// Replaces the reference's two CPU builders:
+//   QBVHScene::Build (src/liblightmetrica/scene.qbvh.cpp:453-591): 12-bin SAH on the
+//     longest centroid axis, two binary levels per 4-wide node, <= 64-triangle leaves;
+//   BVHScene::Build (src/liblightmetrica/scene.bvh.cpp:202-388): pbrt-style 12-bucket SAH.
+// This builder bins all three axes (16 bins by default) with a traversal-cost term,
+// then collapses the binary tree greedily (largest surface area first) into 4-wide
+// nodes laid out for 128-byte cache lines, with leaves of at most 16 triangles
+// stored as contiguous 48-byte vertex records.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace lmgpu {
+
+// One 4-wide node: SoA child boxes (96 B) + child references (16 B) + pad = 128 B.
+struct alignas(128) Node4 {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
+
+// Child reference encoding.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmptyChild = 0xffffffffu;
+constexpr uint32_t kMaxLeafCount = 16;          // 4-bit count field
+constexpr uint32_t kMaxLeafFirst = 0x07ffffffu;  // 27-bit first-record field
+inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+    return kLeafBit | ((count - 1u) << 27) | first;
+}
+
+// Triangle record in leaf order: v0.w carries the scene-order triangle index.
+struct alignas(16) TriRecord {
+    float v0[4];
+    float v1[4];
+    float v2[4];
+};
+static_assert(sizeof(TriRecord) == 48, "TriRecord is 48 bytes");
+
+struct BuildParams {
+    uint32_t max_leaf_size = 4;
+    uint32_t bins = 16;
+    float traversal_cost = 1.2f;
+    int num_threads = 0;
+};
+
+struct WideBVH {
+    std::vector<Node4> nodes;
+    std::vector<TriRecord> tris;
+    uint32_t num_leaves = 0;
+    uint32_t max_depth = 0;  // inner-node levels on the deepest root-to-leaf path
+    double sah_cost = 0.0;
+    double build_seconds = 0.0;
+};
+
+// positions: 3 floats per vertex; faces: 3 indices per triangle.
+// Returns false with `err` set when the input cannot be built.
+bool build_wide_bvh(const float* positions, uint32_t num_vertices, const uint32_t* faces, uint32_t num_triangles,
+                    const BuildParams& params, WideBVH& out, std::string& err);
+
+}  // namespace lmgpu

@@ -1,0 +1,170 @@
+// Internal state of an lmgpu context and the device-side scene records.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lmgpu.h"
+#include "bvh_build.h"
+#include "device_math.hpp"
+
+namespace lmgpu {
+
+// Per-triangle shading record in scene order (64 B): geometric normal (as the reference
+// computes it, normalize(cross(p2 - p1, p3 - p1)), scene.cpp:121-127), material,
+// light, and the three vertex normals for the interpolated shading normal (:130-133).
+struct alignas(16) ShadeTri {
+    float gn[3];
+    uint32_t mat;
+    float n1[3];
+    int32_t light;
+    float n2[3];
+    float pad0;
+    float n3[3];
+    float pad1;
+};
+static_assert(sizeof(ShadeTri) == 64, "ShadeTri is 64 bytes");
+
+struct alignas(16) MaterialDev {
+    uint32_t type;
+    float R[3];
+};
+
+// Area light: its triangles are light_tris[tri_begin, tri_begin + tri_count).
+struct alignas(16) LightDev {
+    uint32_t type;
+    float Le[3];
+    uint32_t tri_begin, tri_count;
+    float inv_area;
+    float pad;
+};
+
+// Light triangle: vertices, w of p1 = cumulative area fraction before it, w of p2 = after it
+// (AreaLight::RegisterPrimitives, light.area.cpp:109-144).
+struct alignas(16) LightTri {
+    float p1[4];
+    float p2[4];
+    float p3[4];
+};
+
+struct CameraDev {
+    M4 view, inv_view, proj, inv_proj;
+    float pos[3];
+    float inv_A;
+    int width, height;
+};
+
+struct EnvDev {
+    float center[3];
+    float radius;
+    float Le[3];
+    float inv_area;
+    int index;  // light index, -1 if there is no environment light
+};
+
+// Kernel classes for lmgpu_kernel_timing.
+enum KernelClass : int {
+    kKTraceClosest = 0,
+    kKTraceAny = 1,
+    kKShade = 2,
+    kKGen = 3,
+    kKShadow = 4,
+    kKExtend = 5,
+    kKNumClasses = 6
+};
+
+template <class T> struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+};
+
+}  // namespace lmgpu
+
+struct lmgpu_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int inject_alloc_failures = 0;
+
+    // Host copies needed by the BVH build and the light / camera setup.
+    std::vector<float> positions;
+    std::vector<uint32_t> faces;
+    uint32_t num_vertices = 0, num_triangles = 0;
+    bool has_scene = false;
+    bool has_shading = false;
+    bool has_camera = false;
+
+    // Device scene
+    lmgpu::DevBuf<lmgpu::Node4> nodes;
+    lmgpu::DevBuf<lmgpu::TriRecord> tris;
+    uint32_t bvh_max_depth = 0;
+    bool has_bvh = false;
+    lmgpu::DevBuf<lmgpu::ShadeTri> shade;
+    lmgpu::DevBuf<lmgpu::MaterialDev> mats;
+    lmgpu::DevBuf<lmgpu::LightDev> lights;
+    lmgpu::DevBuf<lmgpu::LightTri> light_tris;
+    uint32_t num_lights = 0, num_materials = 0;
+    lmgpu::CameraDev cam{};
+    lmgpu::EnvDev env{};
+
+    // Scratch for the trace entry points (staging of host rays / hits).
+    std::vector<void*> scratch;  // freed at destroy
+    void* trace_staging = nullptr;
+    size_t trace_staging_bytes = 0;
+    uint64_t* count_buf = nullptr;  // 2 x u64 for lmgpu_trace_count
+
+    // Path-tracing work buffers (capacity = wavefront width).
+    uint32_t pt_capacity = 0;
+    void* pt_buf = nullptr;
+    size_t pt_buf_bytes = 0;
+    void* film_acc = nullptr;
+    size_t film_acc_bytes = 0;
+    void* film_out_staging = nullptr;
+    size_t film_out_staging_bytes = 0;
+
+    // Timing
+    bool timing = false;
+    double kernel_ms[lmgpu::kKNumClasses] = {};
+    uint64_t kernel_launches[lmgpu::kKNumClasses] = {};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace lmgpu {
+
+// Status helpers: set ctx->err and return a status.
+lmgpu_status fail(lmgpu_ctx* ctx, lmgpu_status s, const std::string& msg);
+lmgpu_status hip_fail(lmgpu_ctx* ctx, hipError_t e, const char* what);
+lmgpu_status dev_alloc(lmgpu_ctx* ctx, void** p, size_t bytes, const char* what);
+void dev_free(void* p);
+
+// Times `launch` with HIP events on the context stream when timing is enabled.
+template <class Fn> hipError_t timed(lmgpu_ctx* ctx, int cls, Fn&& launch) {
+    if (!ctx->timing) {
+        launch();
+        return hipGetLastError();
+    }
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    launch();
+    hipError_t e = hipGetLastError();
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    (void)hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ctx->kernel_ms[cls] += double(ms);
+    ctx->kernel_launches[cls] += 1;
+    return e;
+}
+
+// Kernel launchers (trace_kernels.hip, pt_kernels.hip).
+hipError_t launch_trace_closest(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, float* t, float* b0, float* b1,
+                                int32_t* prim, bool refine);
+hipError_t launch_trace_any(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, uint8_t* occluded);
+hipError_t launch_trace_count(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, unsigned long long* counts);
+lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, lmgpu_pt_stats* stats);
+
+}  // namespace lmgpu

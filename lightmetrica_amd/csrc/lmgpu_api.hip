@@ -1,0 +1,569 @@
+// C ABI of the MI355X path-tracing hot path (include/lmgpu.h).
+#include "context.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+using namespace lmgpu;
+
+namespace lmgpu {
+
+lmgpu_status fail(lmgpu_ctx* ctx, lmgpu_status s, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return s;
+}
+
+lmgpu_status hip_fail(lmgpu_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, LMGPU_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+lmgpu_status dev_alloc(lmgpu_ctx* ctx, void** p, size_t bytes, const char* what) {
+    *p = nullptr;
+    if (ctx->inject_alloc_failures > 0) {
+        ctx->inject_alloc_failures--;
+        return fail(ctx, LMGPU_ERR_INJECTED, std::string("injected allocation failure: ") + what);
+    }
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(ctx, LMGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+    }
+    return LMGPU_OK;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace lmgpu
+
+namespace {
+
+template <class T> void free_buf(DevBuf<T>& b) {
+    dev_free(b.ptr);
+    b.ptr = nullptr;
+    b.count = 0;
+}
+
+template <class T> lmgpu_status upload(lmgpu_ctx* ctx, DevBuf<T>& b, const T* src, size_t count, const char* what) {
+    free_buf(b);
+    void* p = nullptr;
+    lmgpu_status s = dev_alloc(ctx, &p, sizeof(T) * count, what);
+    if (s != LMGPU_OK) return s;
+    b.ptr = static_cast<T*>(p);
+    b.count = count;
+    if (count) {
+        hipError_t e = hipMemcpyAsync(b.ptr, src, sizeof(T) * count, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, what);
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, what);
+    }
+    return LMGPU_OK;
+}
+
+// 4x4 inverse (column-major) by Gauss-Jordan elimination in double, rounded to float.
+bool invert4(const float* m, float* out) {
+    double a[4][8];
+    for (int r = 0; r < 4; r++) {
+        for (int c = 0; c < 4; c++) a[r][c] = m[c * 4 + r];
+        for (int c = 0; c < 4; c++) a[r][4 + c] = r == c ? 1.0 : 0.0;
+    }
+    for (int c = 0; c < 4; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 4; r++)
+            if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+        if (a[piv][c] == 0.0) return false;
+        for (int k = 0; k < 8; k++) std::swap(a[c][k], a[piv][k]);
+        const double inv = 1.0 / a[c][c];
+        for (int k = 0; k < 8; k++) a[c][k] *= inv;
+        for (int r = 0; r < 4; r++) {
+            if (r == c) continue;
+            const double f = a[r][c];
+            for (int k = 0; k < 8; k++) a[r][k] -= f * a[c][k];
+        }
+    }
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) out[c * 4 + r] = float(a[r][4 + c]);
+    return true;
+}
+
+void mul4h(const float* M, const float v[4], float o[4]) {
+    for (int r = 0; r < 4; r++) o[r] = M[r] * v[0] + M[4 + r] * v[1] + M[8 + r] * v[2] + M[12 + r] * v[3];
+}
+
+// PerspectiveCamera::Load (perspectivecamera.cpp:97-172) and RegisterPrimitives (:174-184).
+bool setup_camera(const lmgpu_camera& in, CameraDev& cam) {
+    if (in.width == 0 || in.height == 0) return false;
+    std::memcpy(cam.view.m, in.view, sizeof(float) * 16);
+    if (!invert4(cam.view.m, cam.inv_view.m)) return false;
+    const float aspect = float(in.width) / float(in.height);
+    const float rad = in.fovy_deg * 3.14159265358979323846f / 180.f;
+    const float t = std::tan(float(rad / 2.f));
+    const float zn = 1.f, zf = 1000.f;
+    float* P = cam.proj.m;
+    std::memset(P, 0, sizeof(float) * 16);
+    P[0] = 1.f / (aspect * t);
+    P[5] = 1.f / t;
+    P[10] = -(zf + zn) / (zf - zn);
+    P[11] = -1.f;
+    P[14] = -(2.f * zf * zn) / (zf - zn);
+    if (!invert4(cam.proj.m, cam.inv_proj.m)) return false;
+    const float o[4] = {0.f, 0.f, 0.f, 1.f};
+    float pw[4];
+    mul4h(cam.inv_view.m, o, pw);
+    cam.pos[0] = pw[0];
+    cam.pos[1] = pw[1];
+    cam.pos[2] = pw[2];
+    const float n1[4] = {-1.f, -1.f, 0.f, 1.f}, n2[4] = {1.f, 1.f, 0.f, 1.f};
+    float c1[4], c2[4];
+    mul4h(cam.inv_proj.m, n1, c1);
+    mul4h(cam.inv_proj.m, n2, c2);
+    // The reference divides both corners by the first corner's w (perspectivecamera.cpp:162-163).
+    float p1[3] = {c1[0] / c1[3], c1[1] / c1[3], c1[2] / c1[3]};
+    float p2[3] = {c2[0] / c1[3], c2[1] / c1[3], c2[2] / c1[3]};
+    const float z1 = p1[2], z2 = p2[2];
+    for (int k = 0; k < 3; k++) { p1[k] /= z1; p2[k] /= z2; }
+    const float A = (p2[0] - p1[0]) * (p2[1] - p1[1]);
+    cam.inv_A = 1.f / A;
+    cam.width = int(in.width);
+    cam.height = int(in.height);
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmgpu_abi_version(void) { return LMGPU_ABI_VERSION; }
+
+lmgpu_status lmgpu_create(int device, lmgpu_ctx** out) {
+    if (!out) return LMGPU_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return LMGPU_ERR_INVALID_ARGUMENT;
+    lmgpu_ctx* ctx = new (std::nothrow) lmgpu_ctx();
+    if (!ctx) return LMGPU_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return LMGPU_ERR_HIP;
+    }
+    ctx->stream = ctx->own_stream;
+    ctx->env.index = -1;
+    *out = ctx;
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_destroy(lmgpu_ctx* ctx) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_buf(ctx->nodes);
+    free_buf(ctx->tris);
+    free_buf(ctx->shade);
+    free_buf(ctx->mats);
+    free_buf(ctx->lights);
+    free_buf(ctx->light_tris);
+    dev_free(ctx->trace_staging);
+    dev_free(ctx->count_buf);
+    dev_free(ctx->pt_buf);
+    dev_free(ctx->film_acc);
+    dev_free(ctx->film_out_staging);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return LMGPU_OK;
+}
+
+const char* lmgpu_last_error(const lmgpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+lmgpu_status lmgpu_set_stream(lmgpu_ctx* ctx, void* hip_stream) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_synchronize(lmgpu_ctx* ctx) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? LMGPU_OK : hip_fail(ctx, e, "synchronize");
+}
+
+lmgpu_status lmgpu_inject_alloc_failures(lmgpu_ctx* ctx, int n) {
+    if (!ctx || n < 0) return LMGPU_ERR_INVALID_ARGUMENT;
+    ctx->inject_alloc_failures = n;
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_scene_upload(lmgpu_ctx* ctx, const lmgpu_scene_desc* d) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    if (!d || !d->positions || !d->faces || d->num_vertices == 0 || d->num_triangles == 0)
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: positions and faces are required");
+    (void)hipSetDevice(ctx->device);
+    const uint32_t nv = d->num_vertices, nf = d->num_triangles;
+    for (size_t k = 0; k < size_t(nf) * 3; k++)
+        if (d->faces[k] >= nv) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: face index out of range");
+    const bool shading = d->material_ids != nullptr;
+    if (shading) {
+        if (!d->normals || !d->materials || d->num_materials == 0)
+            return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: materials need normals and a material table");
+        for (uint32_t t = 0; t < nf; t++) {
+            if (d->material_ids[t] >= d->num_materials)
+                return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: material id out of range");
+            if (d->light_ids && (d->light_ids[t] < -1 || d->light_ids[t] >= int32_t(d->num_lights)))
+                return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: light id out of range");
+        }
+        for (uint32_t m = 0; m < d->num_materials; m++)
+            if (d->materials[m].type > LMGPU_BSDF_DIFFUSEMIRROR)
+                return fail(ctx, LMGPU_ERR_UNSUPPORTED, "lmgpu_scene_upload: unsupported BSDF type");
+    }
+    int env_count = 0;
+    for (uint32_t l = 0; l < d->num_lights; l++) {
+        if (d->lights[l].type > LMGPU_LIGHT_ENV_CONST)
+            return fail(ctx, LMGPU_ERR_UNSUPPORTED, "lmgpu_scene_upload: unsupported light type");
+        if (d->lights[l].type == LMGPU_LIGHT_ENV_CONST) env_count++;
+    }
+    if (env_count > 1) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: at most one environment light");
+
+    ctx->positions.assign(d->positions, d->positions + size_t(nv) * 3);
+    ctx->faces.assign(d->faces, d->faces + size_t(nf) * 3);
+    ctx->num_vertices = nv;
+    ctx->num_triangles = nf;
+    ctx->has_bvh = false;
+    ctx->has_shading = false;
+    ctx->has_camera = false;
+    ctx->has_scene = true;
+    free_buf(ctx->nodes);
+    free_buf(ctx->tris);
+
+    if (d->camera) {
+        if (!setup_camera(*d->camera, ctx->cam))
+            return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: invalid camera");
+        ctx->has_camera = true;
+    }
+    if (!shading) return LMGPU_OK;
+
+    const float* P = d->positions;
+    const float* N = d->normals;
+    std::vector<ShadeTri> st(nf);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t t = 0; t < nf; t++) {
+        const uint32_t a = d->faces[3 * t], b = d->faces[3 * t + 1], c = d->faces[3 * t + 2];
+        const float* p1 = P + 3 * size_t(a);
+        const float* p2 = P + 3 * size_t(b);
+        const float* p3 = P + 3 * size_t(c);
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min({lo[k], p1[k], p2[k], p3[k]});
+            hi[k] = std::max({hi[k], p1[k], p2[k], p3[k]});
+        }
+        const float e1[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+        const float e2[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+        float g[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        ShadeTri& s = st[t];
+        for (int k = 0; k < 3; k++) {
+            s.gn[k] = len > 0.f ? g[k] / len : 0.f;
+            s.n1[k] = N[3 * size_t(a) + k];
+            s.n2[k] = N[3 * size_t(b) + k];
+            s.n3[k] = N[3 * size_t(c) + k];
+        }
+        s.mat = d->material_ids[t];
+        s.light = d->light_ids ? d->light_ids[t] : -1;
+        s.pad0 = s.pad1 = 0.f;
+    }
+    lmgpu_status s = upload(ctx, ctx->shade, st.data(), st.size(), "shading records");
+    if (s != LMGPU_OK) return s;
+    std::vector<MaterialDev> mats(d->num_materials);
+    for (uint32_t m = 0; m < d->num_materials; m++) {
+        mats[m].type = d->materials[m].type;
+        for (int k = 0; k < 3; k++) mats[m].R[k] = d->materials[m].R[k];
+    }
+    s = upload(ctx, ctx->mats, mats.data(), mats.size(), "materials");
+    if (s != LMGPU_OK) return s;
+
+    // Lights: area-light CDFs (light.area.cpp:109-144) and the environment sphere
+    // (light.env.const.cpp:88-98 over Scene::GetAABB = triangles + camera, scene.cpp:58-65).
+    std::vector<LightDev> lights(d->num_lights);
+    std::vector<LightTri> ltris;
+    ctx->env.index = -1;
+    for (uint32_t l = 0; l < d->num_lights; l++) {
+        LightDev& L = lights[l];
+        std::memset(&L, 0, sizeof(L));
+        L.type = d->lights[l].type;
+        for (int k = 0; k < 3; k++) L.Le[k] = d->lights[l].Le[k];
+        if (L.type == LMGPU_LIGHT_ENV_CONST) {
+            ctx->env.index = int(l);
+            continue;
+        }
+        L.tri_begin = uint32_t(ltris.size());
+        std::vector<float> cdf(1, 0.f);
+        size_t first = ltris.size();
+        for (uint32_t t = 0; t < nf; t++) {
+            if (!d->light_ids || d->light_ids[t] != int32_t(l)) continue;
+            const float* p1 = P + 3 * size_t(d->faces[3 * t]);
+            const float* p2 = P + 3 * size_t(d->faces[3 * t + 1]);
+            const float* p3 = P + 3 * size_t(d->faces[3 * t + 2]);
+            LightTri lt;
+            for (int k = 0; k < 3; k++) { lt.p1[k] = p1[k]; lt.p2[k] = p2[k]; lt.p3[k] = p3[k]; }
+            lt.p3[3] = 0.f;
+            const float e1[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+            const float e2[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+            const float cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                        cz = e1[0] * e2[1] - e1[1] * e2[0];
+            const float area = std::sqrt(cx * cx + cy * cy + cz * cz) / 2.f;
+            cdf.push_back(cdf.back() + area);
+            ltris.push_back(lt);
+        }
+        L.tri_count = uint32_t(ltris.size() - first);
+        if (L.tri_count == 0)
+            return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: area light without triangles");
+        const float total = cdf.back();
+        L.inv_area = 1.f / total;
+        for (float& v : cdf) v *= L.inv_area;
+        for (uint32_t k = 0; k < L.tri_count; k++) {
+            ltris[first + k].p1[3] = cdf[k];
+            ltris[first + k].p2[3] = cdf[k + 1];
+        }
+    }
+    ctx->num_lights = d->num_lights;
+    ctx->num_materials = d->num_materials;
+    s = upload(ctx, ctx->lights, lights.data(), lights.size(), "lights");
+    if (s != LMGPU_OK) return s;
+    s = upload(ctx, ctx->light_tris, ltris.data(), ltris.size(), "light triangles");
+    if (s != LMGPU_OK) return s;
+    if (ctx->env.index >= 0) {
+        if (!ctx->has_camera)
+            return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: an environment light needs the camera");
+        float blo[3], bhi[3];
+        for (int k = 0; k < 3; k++) {
+            blo[k] = std::min(lo[k], ctx->cam.pos[k]);
+            bhi[k] = std::max(hi[k], ctx->cam.pos[k]);
+        }
+        float r2 = 0.f;
+        for (int k = 0; k < 3; k++) {
+            ctx->env.center[k] = (bhi[k] + blo[k]) / 2.f;
+            const float dk = ctx->env.center[k] - bhi[k];
+            r2 += dk * dk;
+        }
+        ctx->env.radius = std::sqrt(r2);
+        for (int k = 0; k < 3; k++) ctx->env.Le[k] = d->lights[ctx->env.index].Le[k];
+        const float area = 4.f * 3.14159265358979323846f * ctx->env.radius * ctx->env.radius;
+        ctx->env.inv_area = 1.f / area;
+        lights[size_t(ctx->env.index)].inv_area = ctx->env.inv_area;
+        s = upload(ctx, ctx->lights, lights.data(), lights.size(), "lights");
+        if (s != LMGPU_OK) return s;
+    }
+    if (!ctx->has_camera) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: shading scenes need a camera");
+    ctx->has_shading = true;
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_bvh_build(lmgpu_ctx* ctx, const lmgpu_bvh_params* params, lmgpu_bvh_info* info) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return fail(ctx, LMGPU_ERR_NO_SCENE, "lmgpu_bvh_build: upload a scene first");
+    (void)hipSetDevice(ctx->device);
+    BuildParams bp;
+    if (params) {
+        bp.max_leaf_size = params->max_leaf_size ? params->max_leaf_size : 4;
+        bp.bins = params->sah_bins ? params->sah_bins : 16;
+        bp.traversal_cost = params->traversal_cost > 0.f ? params->traversal_cost : 1.2f;
+        bp.num_threads = params->num_threads;
+    }
+    WideBVH w;
+    std::string err;
+    if (!build_wide_bvh(ctx->positions.data(), ctx->num_vertices, ctx->faces.data(), ctx->num_triangles, bp, w, err))
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_bvh_build: " + err);
+    if (3u * w.max_depth > 96u)
+        return fail(ctx, LMGPU_ERR_BVH_TOO_DEEP, "lmgpu_bvh_build: tree depth " + std::to_string(w.max_depth) +
+                                                     " exceeds the traversal stack (max 32)");
+    ctx->has_bvh = false;
+    lmgpu_status s = upload(ctx, ctx->nodes, w.nodes.data(), w.nodes.size(), "BVH nodes");
+    if (s != LMGPU_OK) return s;
+    s = upload(ctx, ctx->tris, w.tris.data(), w.tris.size(), "triangle records");
+    if (s != LMGPU_OK) return s;
+    ctx->bvh_max_depth = w.max_depth;
+    ctx->has_bvh = true;
+    if (info) {
+        info->num_nodes = uint32_t(w.nodes.size());
+        info->num_leaves = w.num_leaves;
+        info->max_depth = w.max_depth;
+        info->num_tri_records = uint32_t(w.tris.size());
+        info->build_seconds = w.build_seconds;
+        info->sah_cost = w.sah_cost;
+        info->device_bytes = uint64_t(w.nodes.size()) * sizeof(Node4) + uint64_t(w.tris.size()) * sizeof(TriRecord);
+    }
+    return LMGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host rays are staged into one device allocation: 8 input arrays + 4 output arrays.
+lmgpu_status stage(lmgpu_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->trace_staging_bytes) return LMGPU_OK;
+    dev_free(ctx->trace_staging);
+    ctx->trace_staging = nullptr;
+    ctx->trace_staging_bytes = 0;
+    lmgpu_status s = dev_alloc(ctx, &ctx->trace_staging, bytes, "ray staging");
+    if (s != LMGPU_OK) return s;
+    ctx->trace_staging_bytes = bytes;
+    return LMGPU_OK;
+}
+
+lmgpu_status prepare_rays(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint32_t n, size_t out_bytes,
+                          const float* dev[8], char** out_base) {
+    const float* src[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    for (int k = 0; k < 8; k++)
+        if (!src[k]) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "ray array is NULL");
+    const size_t in_bytes = rays->device ? 0 : size_t(n) * 4 * 8;
+    lmgpu_status s = stage(ctx, in_bytes + out_bytes);
+    if (s != LMGPU_OK) return s;
+    char* base = static_cast<char*>(ctx->trace_staging);
+    if (rays->device) {
+        for (int k = 0; k < 8; k++) dev[k] = src[k];
+        *out_base = base;
+        return LMGPU_OK;
+    }
+    for (int k = 0; k < 8; k++) {
+        float* d = reinterpret_cast<float*>(base + size_t(k) * n * 4);
+        hipError_t e = hipMemcpyAsync(d, src[k], size_t(n) * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "ray upload");
+        dev[k] = d;
+    }
+    *out_base = base + in_bytes;
+    return LMGPU_OK;
+}
+
+lmgpu_status check_trace(lmgpu_ctx* ctx, const void* rays, const void* out) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    if (!rays || !out) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "trace: NULL rays or output");
+    if (!ctx->has_bvh) return fail(ctx, LMGPU_ERR_NO_BVH, "trace: build the BVH first");
+    (void)hipSetDevice(ctx->device);
+    return LMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lmgpu_status lmgpu_trace_closest(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint32_t n, lmgpu_hits_soa* hits,
+                                 uint32_t flags) {
+    lmgpu_status s = check_trace(ctx, rays, hits);
+    if (s != LMGPU_OK) return s;
+    if (n == 0) return LMGPU_OK;
+    if (!hits->t || !hits->b0 || !hits->b1 || !hits->prim)
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_trace_closest: NULL hit array");
+    const float* dev[8];
+    char* out = nullptr;
+    s = prepare_rays(ctx, rays, n, hits->device ? 0 : size_t(n) * 16, dev, &out);
+    if (s != LMGPU_OK) return s;
+    float *t = hits->t, *b0 = hits->b0, *b1 = hits->b1;
+    int32_t* prim = hits->prim;
+    if (!hits->device) {
+        t = reinterpret_cast<float*>(out);
+        b0 = t + n;
+        b1 = b0 + n;
+        prim = reinterpret_cast<int32_t*>(b1 + n);
+    }
+    hipError_t e = launch_trace_closest(ctx, dev, n, t, b0, b1, prim, (flags & LMGPU_TRACE_REFINE_FP64) != 0);
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_trace_closest");
+    if (!hits->device) {
+        const void* srcs[4] = {t, b0, b1, prim};
+        void* dsts[4] = {hits->t, hits->b0, hits->b1, hits->prim};
+        for (int k = 0; k < 4; k++) {
+            e = hipMemcpyAsync(dsts[k], srcs[k], size_t(n) * 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (e != hipSuccess) return hip_fail(ctx, e, "hit readback");
+        }
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "trace synchronize");
+    }
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_trace_any(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint32_t n, uint8_t* occluded,
+                             int occluded_on_device, uint32_t flags) {
+    (void)flags;
+    lmgpu_status s = check_trace(ctx, rays, occluded);
+    if (s != LMGPU_OK) return s;
+    if (n == 0) return LMGPU_OK;
+    const float* dev[8];
+    char* out = nullptr;
+    s = prepare_rays(ctx, rays, n, occluded_on_device ? 0 : size_t(n), dev, &out);
+    if (s != LMGPU_OK) return s;
+    uint8_t* occ = occluded_on_device ? occluded : reinterpret_cast<uint8_t*>(out);
+    hipError_t e = launch_trace_any(ctx, dev, n, occ);
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_trace_any");
+    if (!occluded_on_device) {
+        e = hipMemcpyAsync(occluded, occ, n, hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "occlusion readback");
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "trace synchronize");
+    }
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_trace_count(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint32_t n, uint64_t* node_visits,
+                               uint64_t* tri_tests) {
+    lmgpu_status s = check_trace(ctx, rays, node_visits);
+    if (s != LMGPU_OK) return s;
+    if (!tri_tests) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_trace_count: NULL output");
+    if (!ctx->count_buf) {
+        void* p = nullptr;
+        s = dev_alloc(ctx, &p, 16, "counters");
+        if (s != LMGPU_OK) return s;
+        ctx->count_buf = static_cast<uint64_t*>(p);
+    }
+    const float* dev[8];
+    char* out = nullptr;
+    s = prepare_rays(ctx, rays, n, 0, dev, &out);
+    if (s != LMGPU_OK) return s;
+    hipError_t e = hipMemsetAsync(ctx->count_buf, 0, 16, ctx->stream);
+    if (e == hipSuccess) e = launch_trace_count(ctx, dev, n, reinterpret_cast<unsigned long long*>(ctx->count_buf));
+    uint64_t h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, ctx->count_buf, 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_trace_count");
+    *node_visits = h[0];
+    *tri_tests = h[1];
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_pt_render(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_film* film, lmgpu_pt_stats* stats) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    if (!params) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: NULL params");
+    (void)hipSetDevice(ctx->device);
+    return run_pt(ctx, params, film, stats);
+}
+
+lmgpu_status lmgpu_enable_timing(lmgpu_ctx* ctx, int enable) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    ctx->timing = enable != 0;
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_kernel_timing(lmgpu_ctx* ctx, int kernel, double* total_ms, uint64_t* launches) {
+    if (!ctx || kernel < 0 || kernel >= kKNumClasses || !total_ms || !launches) return LMGPU_ERR_INVALID_ARGUMENT;
+    *total_ms = ctx->kernel_ms[kernel];
+    *launches = ctx->kernel_launches[kernel];
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_reset_timing(lmgpu_ctx* ctx) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < kKNumClasses; k++) {
+        ctx->kernel_ms[k] = 0.0;
+        ctx->kernel_launches[k] = 0;
+    }
+    return LMGPU_OK;
+}
+
+}  // extern "C"

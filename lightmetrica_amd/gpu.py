@@ -1,0 +1,254 @@
+"""Python host API over the C ABI (include/lmgpu.h).
+
+``Device`` owns one lmgpu context: upload a scene, build the wide BVH, trace ray
+batches (host numpy arrays or device torch tensors) and render with the wavefront
+path tracer.  Every call goes to the HIP kernels of liblmgpu.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _native as N
+from .scenes import FLT_MAX, Rays, Scene
+
+RENDERERS = {"pt": N.LMGPU_PT, "pt.direct": N.LMGPU_PT_DIRECT, "pt.mis": N.LMGPU_PT_MIS}
+FILM_PRECISIONS = {"fp32": N.LMGPU_FILM_FP32, "fp64": N.LMGPU_FILM_FP64,
+                   "fp32c": N.LMGPU_FILM_FP32_COMPENSATED}
+
+# Kernel classes of lmgpu_kernel_timing
+K_TRACE_CLOSEST, K_TRACE_ANY, K_SHADE, K_GEN, K_SHADOW, K_EXTEND = range(6)
+
+
+class LmgpuError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{N.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def _ptr(a: np.ndarray, ctype=C.c_float):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class BvhInfo:
+    num_nodes: int
+    num_leaves: int
+    max_depth: int
+    num_tri_records: int
+    build_seconds: float
+    sah_cost: float
+    device_bytes: int
+
+
+@dataclass
+class RenderStats:
+    closest_rays: int
+    shadow_rays: int
+    samples: int
+    seconds: float
+
+
+class Device:
+    """One lmgpu context on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self._lib = N.load()
+        self._ctx = C.c_void_p()
+        st = self._lib.lmgpu_create(int(device), C.byref(self._ctx))
+        if st != N.LMGPU_OK:
+            raise LmgpuError(st, f"lmgpu_create(device={device}) failed")
+        self._keep = []
+        self.scene: Optional[Scene] = None
+
+    def close(self) -> None:
+        if self._ctx:
+            self._lib.lmgpu_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> None:
+        if st != N.LMGPU_OK:
+            msg = self._lib.lmgpu_last_error(self._ctx)
+            raise LmgpuError(st, msg.decode() if msg else "")
+
+    # ---- scene -------------------------------------------------------------------------
+    def upload(self, scene: Scene) -> None:
+        P = np.ascontiguousarray(scene.positions, dtype=np.float32)
+        Nn = np.ascontiguousarray(scene.normals, dtype=np.float32)
+        F = np.ascontiguousarray(scene.faces, dtype=np.uint32)
+        keep = [P, Nn, F]
+        d = N.SceneDesc()
+        d.positions = _ptr(P)
+        d.normals = _ptr(Nn)
+        d.num_vertices = P.shape[0]
+        d.faces = _ptr(F, C.c_uint32)
+        d.num_triangles = F.shape[0]
+        if scene.material_ids is not None:
+            M = np.ascontiguousarray(scene.material_ids, dtype=np.uint32)
+            L = np.ascontiguousarray(scene.light_ids if scene.light_ids is not None
+                                     else np.full(F.shape[0], -1), dtype=np.int32)
+            mats = (N.Material * len(scene.materials))()
+            for i, m in enumerate(scene.materials):
+                mats[i].type = int(m["type"])
+                mats[i].R[:] = [float(x) for x in m["R"]]
+            nl = 0 if scene.lights is None else len(scene.lights)
+            lights = (N.Light * max(nl, 1))()
+            for i in range(nl):
+                lights[i].type = int(scene.lights[i]["type"])
+                lights[i].Le[:] = [float(x) for x in scene.lights[i]["Le"]]
+            d.material_ids = _ptr(M, C.c_uint32)
+            d.light_ids = _ptr(L, C.c_int32)
+            d.materials = mats
+            d.num_materials = len(scene.materials)
+            d.lights = lights
+            d.num_lights = nl
+            keep += [M, L, mats, lights]
+        if scene.camera is not None:
+            cam = N.CameraDesc()
+            cam.view[:] = [float(x) for x in np.asarray(scene.camera.view, dtype=np.float32)]
+            cam.fovy_deg = float(scene.camera.fovy)
+            cam.width = int(scene.camera.width)
+            cam.height = int(scene.camera.height)
+            d.camera = C.pointer(cam)
+            keep.append(cam)
+        self._check(self._lib.lmgpu_scene_upload(self._ctx, C.byref(d)))
+        self.scene = scene
+
+    def build_bvh(self, max_leaf_size: int = 4, sah_bins: int = 16, traversal_cost: float = 1.2,
+                  num_threads: int = 0) -> BvhInfo:
+        p = N.BvhParams(max_leaf_size, sah_bins, traversal_cost, num_threads)
+        info = N.BvhInfo()
+        self._check(self._lib.lmgpu_bvh_build(self._ctx, C.byref(p), C.byref(info)))
+        return BvhInfo(info.num_nodes, info.num_leaves, info.max_depth, info.num_tri_records, info.build_seconds,
+                       info.sah_cost, info.device_bytes)
+
+    # ---- traversal (host arrays) ----------------------------------------------------
+    @staticmethod
+    def _host_rays(rays: Rays):
+        cols = [np.ascontiguousarray(rays.o[:, k], dtype=np.float32) for k in range(3)]
+        cols += [np.ascontiguousarray(rays.d[:, k], dtype=np.float32) for k in range(3)]
+        cols += [np.ascontiguousarray(rays.tmin, dtype=np.float32), np.ascontiguousarray(rays.tmax, dtype=np.float32)]
+        soa = N.RaysSoA(*[c.ctypes.data for c in cols], 0)
+        return soa, cols
+
+    def trace_closest(self, rays: Rays, refine_fp64: bool = True):
+        """Closest hits of host rays: (t, b0, b1, prim) numpy arrays; prim = -1 for a miss."""
+        n = len(rays)
+        soa, keep = self._host_rays(rays)
+        t = np.empty(n, np.float32)
+        b0 = np.empty(n, np.float32)
+        b1 = np.empty(n, np.float32)
+        prim = np.empty(n, np.int32)
+        hits = N.HitsSoA(t.ctypes.data, b0.ctypes.data, b1.ctypes.data, prim.ctypes.data, 0)
+        flags = N.LMGPU_TRACE_REFINE_FP64 if refine_fp64 else 0
+        self._check(self._lib.lmgpu_trace_closest(self._ctx, C.byref(soa), n, C.byref(hits), flags))
+        return t, b0, b1, prim
+
+    def trace_any(self, rays: Rays) -> np.ndarray:
+        n = len(rays)
+        soa, keep = self._host_rays(rays)
+        occ = np.empty(n, np.uint8)
+        self._check(self._lib.lmgpu_trace_any(self._ctx, C.byref(soa), n, occ.ctypes.data, 0, 0))
+        return occ.astype(bool)
+
+    def trace_count(self, rays: Rays):
+        """(node visits, triangle tests) summed over the rays, from the counting traversal."""
+        soa, keep = self._host_rays(rays)
+        nv, nt = C.c_uint64(), C.c_uint64()
+        self._check(self._lib.lmgpu_trace_count(self._ctx, C.byref(soa), len(rays), C.byref(nv), C.byref(nt)))
+        return int(nv.value), int(nt.value)
+
+    # ---- traversal (device tensors) -------------------------------------------------
+    def set_stream(self, stream_ptr: int) -> None:
+        self._check(self._lib.lmgpu_set_stream(self._ctx, C.c_void_p(stream_ptr)))
+
+    def trace_closest_device(self, ray_cols, out_cols, n: int, refine_fp64: bool = False) -> None:
+        """Device variant: ray_cols = 8 device pointers (ox..tmax), out_cols = 4 (t, b0, b1, prim)."""
+        soa = N.RaysSoA(*[int(p) for p in ray_cols], 1)
+        hits = N.HitsSoA(*[int(p) for p in out_cols], 1)
+        flags = N.LMGPU_TRACE_REFINE_FP64 if refine_fp64 else 0
+        self._check(self._lib.lmgpu_trace_closest(self._ctx, C.byref(soa), n, C.byref(hits), flags))
+
+    def trace_any_device(self, ray_cols, occluded_ptr: int, n: int) -> None:
+        soa = N.RaysSoA(*[int(p) for p in ray_cols], 1)
+        self._check(self._lib.lmgpu_trace_any(self._ctx, C.byref(soa), n, C.c_void_p(int(occluded_ptr)), 1, 0))
+
+    def synchronize(self) -> None:
+        self._check(self._lib.lmgpu_synchronize(self._ctx))
+
+    # ---- path tracing ----------------------------------------------------------------
+    def render(self, renderer: str = "pt", num_samples: Optional[int] = None, max_path_vertices: Optional[int] = None,
+               rr_depth: Optional[int] = None, seed: Optional[int] = None, sample_offset: int = 0,
+               wavefront_width: int = 0, film_precision: str = "fp32", out_precision: str = "fp64",
+               device_out_ptr: Optional[int] = None):
+        """Render the uploaded scene.  Returns (film (H, W, 3) or None for device output, RenderStats).
+
+        Film rows are bottom-up (raster y = 0 is the bottom row), values rescaled by W*H/N.
+        """
+        sc = self.scene
+        if sc is None or sc.camera is None:
+            raise LmgpuError(3, "upload a scene with a camera first")
+        rc = sc.render
+        p = N.PtParams()
+        p.renderer = RENDERERS[renderer]
+        p.num_samples = int(num_samples if num_samples is not None else rc.num_samples)
+        p.max_path_vertices = int(max_path_vertices if max_path_vertices is not None else rc.max_path_vertices)
+        p.rr_depth = int(rr_depth if rr_depth is not None else rc.rr_depth)
+        p.seed = int(seed if seed is not None else rc.seed)
+        p.sample_offset = int(sample_offset)
+        p.wavefront_width = int(wavefront_width)
+        p.film_precision = FILM_PRECISIONS[film_precision]
+        W, H = sc.camera.width, sc.camera.height
+        film = N.Film()
+        film.precision = N.LMGPU_FILM_FP64 if out_precision == "fp64" else N.LMGPU_FILM_FP32
+        film.width, film.height = W, H
+        out = None
+        if device_out_ptr is not None:
+            film.data = int(device_out_ptr)
+            film.device = 1
+        else:
+            out = np.empty((H, W, 3), dtype=np.float64 if out_precision == "fp64" else np.float32)
+            film.data = out.ctypes.data
+            film.device = 0
+        stats = N.PtStats()
+        self._check(self._lib.lmgpu_pt_render(self._ctx, C.byref(p), C.byref(film), C.byref(stats)))
+        return out, RenderStats(int(stats.closest_rays), int(stats.shadow_rays), int(stats.samples),
+                                float(stats.seconds))
+
+    # ---- timing / test hooks ---------------------------------------------------------
+    def enable_timing(self, on: bool = True) -> None:
+        self._check(self._lib.lmgpu_enable_timing(self._ctx, int(on)))
+
+    def reset_timing(self) -> None:
+        self._check(self._lib.lmgpu_reset_timing(self._ctx))
+
+    def kernel_timing(self, kernel: int):
+        ms, n = C.c_double(), C.c_uint64()
+        self._check(self._lib.lmgpu_kernel_timing(self._ctx, kernel, C.byref(ms), C.byref(n)))
+        return float(ms.value), int(n.value)
+
+    def inject_alloc_failures(self, n: int) -> None:
+        self._check(self._lib.lmgpu_inject_alloc_failures(self._ctx, n))
+
+
+def rays_from_arrays(o, d, tmin=None, tmax=None) -> Rays:
+    o = np.asarray(o, np.float32)
+    d = np.asarray(d, np.float32)
+    n = o.shape[0]
+    return Rays(o, d, np.zeros(n, np.float32) if tmin is None else np.asarray(tmin, np.float32),
+                np.full(n, FLT_MAX, np.float32) if tmax is None else np.asarray(tmax, np.float32))

@@ -1,0 +1,1466 @@
+// lmref -- CPU restatement of Lightmetrica's path-tracing hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py load this code, and only as the checker or as the
+// timed CPU baseline.  The product (lightmetrica_amd, liblmgpu.so) never links,
+// loads or calls it.
+//
+// Everything here is templated on the arithmetic type F (float or double), the way
+// the reference is templated on Math::Float (include/lightmetrica/math.types.h:36-46):
+//   F = float : the FP32 build (QBVH + SSE quad Moeller-Trumbore, rsqrt normalize,
+//               Eps 1e-4).  Used as the CPU speed baseline.
+//   F = double: the FP64 oracle (binary SAH BVH + TriAccel, exact normalize,
+//               Eps 1e-7).  The reference compiles the QBVH out in double mode
+//               (src/liblightmetrica/scene.qbvh.cpp:35), so the FP64 oracle is BVH.
+//
+// Each function cites the reference file:line it restates.  Paths are relative to
+// the reference repository root.
+//
+// Parity pinning of each part is listed in oracle/README.md.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include <immintrin.h>
+
+namespace lmref {
+
+// ---------------------------------------------------------------------------------
+// Constants -- include/lightmetrica/math.constants.inl:51-79
+// Inf() is numeric_limits<T>::max(), not +infinity.
+
+template <class F> struct K;
+template <> struct K<float> {
+    static float eps() { return 1e-4f; }
+    static float eps_large() { return 1e-3f; }
+    static float inf() { return FLT_MAX; }
+};
+template <> struct K<double> {
+    static double eps() { return 1e-7; }
+    static double eps_large() { return 1e-5; }
+    static double inf() { return DBL_MAX; }
+};
+template <class F> inline F kPi() { return F(3.141592653589793238462643383279502884); }
+template <class F> inline F kInvPi() { return F(0.318309886183790671537767526745028724); }
+
+// When true, float Normalize uses the SSE rsqrt approximation exactly as the
+// reference's Vec3f specialisation does (include/lightmetrica/math.vector.inl:973-981).
+// When false, float Normalize divides by the exact length (the double path,
+// math.vector.inl:1407-1411).  Set once before a render; not per-thread.
+extern bool g_approx_rsqrt;
+
+// ---------------------------------------------------------------------------------
+// Vectors -- include/lightmetrica/math.vector.h (generic TVec2/TVec3/TVec4)
+
+template <class F> struct V2 {
+    F x, y;
+    V2() : x(0), y(0) {}
+    V2(F a, F b) : x(a), y(b) {}
+    F operator[](int i) const { return i == 0 ? x : y; }
+};
+
+template <class F> struct V3 {
+    F x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    explicit V3(F s) : x(s), y(s), z(s) {}
+    V3(F a, F b, F c) : x(a), y(b), z(c) {}
+    F operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    F& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    V3 operator+(const V3& o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    V3 operator-(const V3& o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    V3 operator-() const { return V3(-x, -y, -z); }
+    V3 operator*(const V3& o) const { return V3(x * o.x, y * o.y, z * o.z); }
+    V3 operator*(F s) const { return V3(x * s, y * s, z * s); }
+    V3 operator/(F s) const { return V3(x / s, y / s, z / s); }
+    V3& operator+=(const V3& o) { x += o.x; y += o.y; z += o.z; return *this; }
+    V3& operator*=(const V3& o) { x *= o.x; y *= o.y; z *= o.z; return *this; }
+    V3& operator/=(F s) { x /= s; y /= s; z /= s; return *this; }
+    bool operator!=(const V3& o) const { return x != o.x || y != o.y || z != o.z; }
+};
+template <class F> inline V3<F> operator*(F s, const V3<F>& v) { return v * s; }
+
+template <class F> struct V4 {
+    F x, y, z, w;
+    V4() : x(0), y(0), z(0), w(0) {}
+    V4(F a, F b, F c, F d) : x(a), y(b), z(c), w(d) {}
+    V4(const V3<F>& v, F d) : x(v.x), y(v.y), z(v.z), w(d) {}
+    F operator[](int i) const { return i == 0 ? x : (i == 1 ? y : (i == 2 ? z : w)); }
+    F& operator[](int i) { return i == 0 ? x : (i == 1 ? y : (i == 2 ? z : w)); }
+    V3<F> xyz() const { return V3<F>(x, y, z); }
+};
+
+template <class F> inline F dot(const V3<F>& a, const V3<F>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class F> inline V3<F> cross(const V3<F>& a, const V3<F>& b) {
+    return V3<F>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <class F> inline F length2(const V3<F>& v) { return dot(v, v); }
+template <class F> inline F length(const V3<F>& v) { return std::sqrt(length2(v)); }
+template <class F> inline V3<F> vmin(const V3<F>& a, const V3<F>& b) {
+    return V3<F>(std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z));
+}
+template <class F> inline V3<F> vmax(const V3<F>& a, const V3<F>& b) {
+    return V3<F>(std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z));
+}
+template <class F> inline bool is_zero(const V3<F>& v) { return v.x == F(0) && v.y == F(0) && v.z == F(0); }
+
+// Normalize -- math.vector.inl:973-981 (float, SSE rsqrt) and :1407-1411 (double).
+inline V3<float> normalize_impl(const V3<float>& v) {
+    if (g_approx_rsqrt) {
+        __m128 a = _mm_set_ps(0.f, v.z, v.y, v.x);
+        __m128 d = _mm_dp_ps(a, a, 0x7f);
+        __m128 r = _mm_mul_ps(a, _mm_rsqrt_ps(d));
+        alignas(16) float o[4];
+        _mm_store_ps(o, r);
+        return V3<float>(o[0], o[1], o[2]);
+    }
+    return v / length(v);
+}
+inline V3<double> normalize_impl(const V3<double>& v) { return v / length(v); }
+template <class F> inline V3<F> normalize(const V3<F>& v) { return normalize_impl(v); }
+
+// Luminance -- math.vector.inl:492-499
+template <class F> inline F luminance(const V3<F>& v) {
+    return F(0.212671) * v.x + F(0.715160) * v.y + F(0.072169) * v.z;
+}
+// ReflectZUp -- math.vector.inl:532-536
+template <class F> inline V3<F> reflect_zup(const V3<F>& w) { return V3<F>(-w.x, -w.y, w.z); }
+
+// ---------------------------------------------------------------------------------
+// 4x4 matrices, column-major: c[i] is column i (include/lightmetrica/math.matrix.inl:343-350)
+
+template <class F> struct M4 {
+    V4<F> c[4];
+    static M4 identity() {
+        M4 m;
+        for (int i = 0; i < 4; i++) m.c[i][i] = F(1);
+        return m;
+    }
+    V4<F> operator*(const V4<F>& v) const {
+        return V4<F>(c[0].x * v.x + c[1].x * v.y + c[2].x * v.z + c[3].x * v.w,
+                     c[0].y * v.x + c[1].y * v.y + c[2].y * v.z + c[3].y * v.w,
+                     c[0].z * v.x + c[1].z * v.y + c[2].z * v.z + c[3].z * v.w,
+                     c[0].w * v.x + c[1].w * v.y + c[2].w * v.z + c[3].w * v.w);
+    }
+    F at(int row, int col) const { return c[col][row]; }
+};
+
+// General inverse by cofactor expansion (math.matrix.inl:375 computes the same
+// matrix; the operation order here is this file's own).
+template <class F> M4<F> inverse(const M4<F>& m) {
+    F a[16];
+    for (int r = 0; r < 4; r++)
+        for (int col = 0; col < 4; col++) a[r * 4 + col] = m.at(r, col);
+    F inv[16];
+    auto minor3 = [&](int r0, int r1, int r2, int c0, int c1, int c2) {
+        return a[r0 * 4 + c0] * (a[r1 * 4 + c1] * a[r2 * 4 + c2] - a[r1 * 4 + c2] * a[r2 * 4 + c1]) -
+               a[r0 * 4 + c1] * (a[r1 * 4 + c0] * a[r2 * 4 + c2] - a[r1 * 4 + c2] * a[r2 * 4 + c0]) +
+               a[r0 * 4 + c2] * (a[r1 * 4 + c0] * a[r2 * 4 + c1] - a[r1 * 4 + c1] * a[r2 * 4 + c0]);
+    };
+    for (int r = 0; r < 4; r++) {
+        for (int col = 0; col < 4; col++) {
+            int rr[3], cc[3], k = 0, l = 0;
+            for (int i = 0; i < 4; i++) if (i != r) rr[k++] = i;
+            for (int j = 0; j < 4; j++) if (j != col) cc[l++] = j;
+            F cof = minor3(rr[0], rr[1], rr[2], cc[0], cc[1], cc[2]);
+            if ((r + col) & 1) cof = -cof;
+            inv[col * 4 + r] = cof;  // adjugate = transpose of cofactors
+        }
+    }
+    F det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+    M4<F> out;
+    for (int r = 0; r < 4; r++)
+        for (int col = 0; col < 4; col++) out.c[col][r] = inv[r * 4 + col] / det;
+    return out;
+}
+
+// LookAt -- include/lightmetrica/math.transform.inl:87-100
+template <class F> M4<F> look_at(const V3<F>& eye, const V3<F>& center, const V3<F>& up) {
+    V3<F> f = normalize(center - eye);
+    V3<F> u = normalize(up);
+    V3<F> s = normalize(cross(f, u));
+    u = cross(s, f);
+    M4<F> m;
+    m.c[0] = V4<F>(s.x, u.x, -f.x, F(0));
+    m.c[1] = V4<F>(s.y, u.y, -f.y, F(0));
+    m.c[2] = V4<F>(s.z, u.z, -f.z, F(0));
+    m.c[3] = V4<F>(-dot(s, eye), -dot(u, eye), dot(f, eye), F(1));
+    return m;
+}
+
+// Perspective -- math.transform.inl:102-113 (fovy in degrees)
+template <class F> M4<F> perspective(F fovy, F aspect, F znear, F zfar) {
+    F rad = fovy * kPi<F>() / F(180);
+    F t = std::tan(F(rad / F(2)));
+    M4<F> m;
+    m.c[0] = V4<F>(F(1) / (aspect * t), 0, 0, 0);
+    m.c[1] = V4<F>(0, F(1) / t, 0, 0);
+    m.c[2] = V4<F>(0, 0, -(zfar + znear) / (zfar - znear), F(-1));
+    m.c[3] = V4<F>(0, 0, -(F(2) * zfar * znear) / (zfar - znear), 0);
+    return m;
+}
+
+// ---------------------------------------------------------------------------------
+// Sampling warps -- include/lightmetrica/math.stats.inl
+
+// Shirley-Chiu concentric map -- math.stats.inl:25-63
+template <class F> V2<F> concentric_disk(const V2<F>& u) {
+    F v1 = F(2) * u.x - F(1);
+    F v2 = F(2) * u.y - F(1);
+    F r, th;
+    const F q = kPi<F>() / F(4);
+    if (v1 == F(0) && v2 == F(0)) {
+        r = 0; th = 0;
+    } else if (v1 > -v2) {
+        if (v1 > v2) { r = v1; th = q * v2 / v1; }
+        else { r = v2; th = q * (F(2) - v1 / v2); }
+    } else {
+        if (v1 < v2) { r = -v1; th = q * (F(4) + v2 / v1); }
+        else { r = -v2; th = q * (F(6) - v1 / v2); }
+    }
+    return V2<F>(r * std::cos(th), r * std::sin(th));
+}
+// Malley -- math.stats.inl:71-76
+template <class F> V3<F> cosine_hemisphere(const V2<F>& u) {
+    V2<F> s = concentric_disk(u);
+    return V3<F>(s.x, s.y, std::sqrt(std::max(F(0), F(1) - s.x * s.x - s.y * s.y)));
+}
+// math.stats.inl:105-112
+template <class F> V3<F> uniform_sphere(const V2<F>& u) {
+    F z = F(1) - F(2) * u.x;
+    F r = std::sqrt(std::max(F(0), F(1) - z * z));
+    F phi = F(2) * kPi<F>() * u.y;
+    return V3<F>(r * std::cos(phi), r * std::sin(phi), z);
+}
+// math.stats.inl:120-125
+template <class F> V2<F> uniform_triangle(const V2<F>& u) {
+    F s = std::sqrt(std::max(F(0), u.x));
+    return V2<F>(F(1) - s, u.y * s);
+}
+
+// ---------------------------------------------------------------------------------
+// Random numbers.
+//
+// Random::Next = Float(NextUInt() * 2^-32) -- include/lightmetrica/random.inl:24-27
+// (computed in double, then rounded to Float; can round to 1.0f).
+//
+// Streams.  The reference draws from one sequential SFMT19937 stream per thread
+// (src/liblightmetrica/randomsampler.cpp:42-99, sfmtrand.cpp:35-51).  SFMT is a
+// third-party library absent here, so two stream kinds are restated:
+//   * MT: std::mt19937 -- the reference's own 'mt' option (standardmtrand.cpp:30-47),
+//     per-thread sequential, seeded like PathtraceRenderer::CreateRenderProcess
+//     (pathtrace.cpp:156-161).  Used for the timed CPU baseline.
+//   * Philox: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), one stream per
+//     path sample keyed by (seed, global sample index), consumed sequentially in
+//     exactly the order the reference estimator calls Next()/NextVec2().  The GPU
+//     product uses the same streams, so the GPU and this oracle trace the same
+//     paths for the same sample index.
+
+struct Philox4x32 {
+    static inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+        uint64_t p = uint64_t(a) * uint64_t(b);
+        hi = uint32_t(p >> 32);
+        lo = uint32_t(p);
+    }
+    // 10 rounds; key bumped by the Weyl constants between rounds.
+    static inline void block(const uint32_t key_in[2], const uint32_t ctr_in[4], uint32_t out[4]) {
+        uint32_t k0 = key_in[0], k1 = key_in[1];
+        uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+        for (int r = 0; r < 10; r++) {
+            uint32_t hi0, lo0, hi1, lo1;
+            mulhilo(0xD2511F53u, c0, hi0, lo0);
+            mulhilo(0xCD9E8D57u, c2, hi1, lo1);
+            uint32_t n0 = hi1 ^ c1 ^ k0;
+            uint32_t n2 = hi0 ^ c3 ^ k1;
+            c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+    }
+};
+
+// Stream tag in counter word 3: 'LMPT'.
+constexpr uint32_t kStreamTag = 0x4c4d5054u;
+
+struct RngSource {
+    virtual ~RngSource() {}
+    virtual uint32_t next_u32() = 0;
+    virtual void begin_sample(uint64_t /*global_sample_index*/) {}
+};
+
+struct PhiloxSource : RngSource {
+    uint32_t key[2];
+    uint64_t sample = 0;
+    uint32_t dim = 0;
+    uint32_t cache[4];
+    uint32_t cached_block = 0xffffffffu;
+    explicit PhiloxSource(uint64_t seed) {
+        key[0] = uint32_t(seed);
+        key[1] = uint32_t(seed >> 32);
+    }
+    void begin_sample(uint64_t s) override {
+        sample = s;
+        dim = 0;
+        cached_block = 0xffffffffu;
+    }
+    uint32_t next_u32() override {
+        uint32_t b = dim >> 2;
+        if (b != cached_block) {
+            uint32_t ctr[4] = {b, uint32_t(sample), uint32_t(sample >> 32), kStreamTag};
+            Philox4x32::block(key, ctr, cache);
+            cached_block = b;
+        }
+        return cache[dim++ & 3u];
+    }
+};
+
+struct MTSource : RngSource {
+    std::mt19937 g;
+    explicit MTSource(uint32_t seed) : g(seed) {}
+    uint32_t next_u32() override { return uint32_t(g()); }
+};
+
+template <class F> struct Sampler {
+    RngSource* src;
+    F next() { return F(double(src->next_u32()) * (1.0 / 4294967296.0)); }
+    V2<F> next2() {  // random.inl:30-37: u1 first, then u2
+        F u1 = next();
+        F u2 = next();
+        return V2<F>(u1, u2);
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// AABB -- include/lightmetrica/aabb.inl:20-103.  Empty = (Inf, -Inf) with Inf = FLT_MAX/DBL_MAX.
+
+template <class F> struct AABB {
+    V3<F> mn, mx;
+    AABB() : mn(K<F>::inf()), mx(-K<F>::inf()) {}
+    AABB(const V3<F>& p) : mn(p), mx(p) {}
+    AABB(const V3<F>& a, const V3<F>& b) : mn(vmin(a, b)), mx(vmax(a, b)) {}
+    AABB uni(const AABB& b) const { AABB r; r.mn = vmin(mn, b.mn); r.mx = vmax(mx, b.mx); return r; }
+    AABB uni(const V3<F>& p) const { AABB r; r.mn = vmin(mn, p); r.mx = vmax(mx, p); return r; }
+    F surface_area() const {
+        V3<F> d = mx - mn;
+        return F(2) * (d.x * d.y + d.y * d.z + d.z * d.x);
+    }
+    int longest_axis() const {  // ties go to z (aabb.inl:69-73)
+        V3<F> d = mx - mn;
+        return d.x > d.y && d.x > d.z ? 0 : (d.y > d.z ? 1 : 2);
+    }
+    const V3<F>& operator[](int i) const { return i == 0 ? mn : mx; }
+};
+
+// ---------------------------------------------------------------------------------
+// Ray -- include/lightmetrica/ray.h
+
+template <class F> struct Ray {
+    V3<F> o, d;
+    F minT, maxT;
+};
+
+// Triangle hit produced by an accelerator (before hit reconstruction).
+template <class F> struct TriHit {
+    uint32_t tri;  // triangle index in scene order
+    F b0, b1;      // weights of vertex 2 and vertex 3 (scene.cpp:133)
+};
+
+// ---------------------------------------------------------------------------------
+// TriAccel (Wald projection test) -- include/lightmetrica/triaccel.h:31-160
+
+template <class F> struct TriAccel {
+    uint32_t k;
+    F n_u, n_v, n_d, a_u, a_v, b_nu, b_nv, c_nu, c_nv;
+    uint32_t tri;
+
+    // triaccel.h:57-98
+    int load(const V3<F>& A, const V3<F>& B, const V3<F>& C) {
+        static const int wald_modulo[4] = {1, 2, 0, 1};
+        V3<F> b = C - A;
+        V3<F> c = B - A;
+        V3<F> N = cross(c, b);
+        k = 0;
+        for (int j = 0; j < 3; j++)
+            if (std::abs(N[j]) > std::abs(N[k])) k = j;
+        uint32_t u = wald_modulo[k], v = wald_modulo[k + 1];
+        const F n_k = N[k];
+        const F denom = b[u] * c[v] - b[v] * c[u];
+        if (denom == F(0)) { k = 3; return 1; }
+        n_u = N[u] / n_k;
+        n_v = N[v] / n_k;
+        n_d = dot(A, N) / n_k;
+        b_nu = b[u] / denom;
+        b_nv = -b[v] / denom;
+        a_u = A[u];
+        a_v = A[v];
+        c_nu = c[v] / denom;
+        c_nv = -c[u] / denom;
+        return 0;
+    }
+
+    // triaccel.h:100-160 -- accepts t in [mint, maxt] inclusive.
+    bool intersect(const Ray<F>& ray, F mint, F maxt, F& u, F& v, F& t) const {
+        F o_u, o_v, o_k, d_u, d_v, d_k;
+        switch (k) {
+            case 0: o_u = ray.o.y; o_v = ray.o.z; o_k = ray.o.x; d_u = ray.d.y; d_v = ray.d.z; d_k = ray.d.x; break;
+            case 1: o_u = ray.o.z; o_v = ray.o.x; o_k = ray.o.y; d_u = ray.d.z; d_v = ray.d.x; d_k = ray.d.y; break;
+            case 2: o_u = ray.o.x; o_v = ray.o.y; o_k = ray.o.z; d_u = ray.d.x; d_v = ray.d.y; d_k = ray.d.z; break;
+            default: return false;
+        }
+        t = (n_d - o_u * n_u - o_v * n_v - o_k) / (d_u * n_u + d_v * n_v + d_k);
+        if (t < mint || t > maxt) return false;
+        const F hu = o_u + t * d_u - a_u;
+        const F hv = o_v + t * d_v - a_v;
+        u = hv * b_nu + hu * b_nv;
+        v = hu * c_nu + hv * c_nv;
+        return u >= F(0) && v >= F(0) && u + v <= F(1);
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Accelerator interface (Scene::IntersectTriangles, include/lightmetrica/scene.h:156)
+
+template <class F> struct Accel {
+    virtual ~Accel() {}
+    // Closest hit.  On success ray.maxT is the hit distance (the reference updates
+    // the ray in place, scene.qbvh.cpp:111-122, scene.bvh.cpp:418-426).
+    virtual bool intersect(Ray<F>& ray, TriHit<F>& hit) const = 0;
+    virtual const char* name() const = 0;
+};
+
+// Triangle positions in F, scene order.
+template <class F> struct TriSoup {
+    std::vector<V3<F>> p1, p2, p3;
+    size_t size() const { return p1.size(); }
+};
+
+// NaiveScene -- src/liblightmetrica/scene.naive.cpp:31-124 (last-wins on equal t).
+template <class F> struct NaiveAccel : Accel<F> {
+    std::vector<TriAccel<F>> ta;
+    explicit NaiveAccel(const TriSoup<F>& s) {
+        ta.resize(s.size());
+        for (size_t i = 0; i < s.size(); i++) { ta[i].load(s.p1[i], s.p2[i], s.p3[i]); ta[i].tri = uint32_t(i); }
+    }
+    bool intersect(Ray<F>& ray, TriHit<F>& hit) const override {
+        bool any = false;
+        for (size_t i = 0; i < ta.size(); i++) {
+            F t, u, v;
+            if (ta[i].intersect(ray, ray.minT, ray.maxT, u, v, t)) {
+                ray.maxT = t;
+                hit.tri = ta[i].tri; hit.b0 = u; hit.b1 = v;
+                any = true;
+            }
+        }
+        return any;
+    }
+    const char* name() const override { return "naive"; }
+};
+
+// BVHScene -- src/liblightmetrica/scene.bvh.cpp:158-485.  pbrt-style binary SAH with
+// 12 buckets, cost 0.125 + (n1 A1 + n2 A2)/A, leaf when cost >= n and n <= 255.
+template <class F> struct BVHAccel : Accel<F> {
+    struct Node {
+        AABB<F> bound;
+        int left = -1, right = -1;  // children (internal)
+        int axis = -1;
+        int begin = 0, end = 0;     // leaf range in idx
+        bool leaf = false;
+    };
+    std::vector<TriAccel<F>> ta;
+    std::vector<int> idx;
+    std::vector<Node> nodes;
+    std::vector<AABB<F>> tb;
+    std::vector<V3<F>> tc;
+    static constexpr int kMaxTriInNode = 255;
+
+    explicit BVHAccel(const TriSoup<F>& s) {
+        size_t n = s.size();
+        ta.resize(n); tb.resize(n); tc.resize(n); idx.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            ta[i].load(s.p1[i], s.p2[i], s.p3[i]);
+            ta[i].tri = uint32_t(i);
+            idx[i] = int(i);
+            AABB<F> b(s.p1[i], s.p2[i]);
+            b = b.uni(s.p3[i]);
+            tb[i] = b;
+            tc[i] = (b.mn + b.mx) * F(0.5);
+        }
+        nodes.reserve(2 * n + 1);
+        if (n > 0) build(0, int(n));
+    }
+
+    // scene.bvh.cpp:236-388
+    int build(int begin, int end) {
+        AABB<F> bound;
+        for (int i = begin; i < end; i++) bound = bound.uni(tb[idx[i]]);
+        int node = int(nodes.size());
+        nodes.push_back(Node());
+        nodes[node].bound = bound;
+        int n = end - begin;
+        auto make_leaf = [&]() { nodes[node].leaf = true; nodes[node].begin = begin; nodes[node].end = end; };
+        if (n == 1) { make_leaf(); return node; }
+        AABB<F> cb;
+        for (int i = begin; i < end; i++) cb = cb.uni(tc[idx[i]]);
+        int axis = cb.longest_axis();
+        if (cb.mn[axis] == cb.mx[axis]) { make_leaf(); return node; }
+        const int nb = 12;
+        AABB<F> bb[nb];
+        int bc[nb] = {0};
+        auto bucket_of = [&](int t) {
+            int b = int(F(F(nb) * ((tc[t][axis] - cb.mn[axis]) / (cb.mx[axis] - cb.mn[axis]))));
+            return std::min(b, nb - 1);
+        };
+        for (int i = begin; i < end; i++) {
+            int b = bucket_of(idx[i]);
+            bc[b]++;
+            bb[b] = bb[b].uni(tb[idx[i]]);
+        }
+        F costs[nb - 1];
+        for (int i = 0; i < nb - 1; i++) {
+            AABB<F> b1, b2;
+            int c1 = 0, c2 = 0;
+            for (int j = 0; j <= i; j++) { b1 = b1.uni(bb[j]); c1 += bc[j]; }
+            for (int j = i + 1; j < nb; j++) { b2 = b2.uni(bb[j]); c2 += bc[j]; }
+            costs[i] = F(0.125) + (F(c1) * b1.surface_area() + F(c2) * b2.surface_area()) / bound.surface_area();
+        }
+        int best = 0;
+        double min_cost = costs[0];
+        for (int i = 1; i < nb - 1; i++)
+            if (min_cost > costs[i]) { min_cost = costs[i]; best = i; }
+        if (min_cost < double(n) || n > kMaxTriInNode) {
+            int* mid_ptr = std::partition(&idx[begin], &idx[end - 1] + 1, [&](int t) { return bucket_of(t) <= best; });
+            int mid = int(mid_ptr - &idx[0]);
+            if (mid == begin || mid == end) { make_leaf(); return node; }  // guard: see README
+            int l = build(begin, mid);
+            int r = build(mid, end);
+            nodes[node].left = l; nodes[node].right = r; nodes[node].axis = axis;
+            return node;
+        }
+        make_leaf();
+        return node;
+    }
+
+    struct Trav {
+        Ray<F>& ray;
+        int neg[3];
+        V3<F> inv_min, inv_max;
+        TriHit<F> hit;
+    };
+
+    // scene.bvh.cpp:450-473 (slab test with early exit, strict at the end)
+    bool box(const AABB<F>& b, const Trav& d) const {
+        const Ray<F>& r = d.ray;
+        F tmin = (b[d.neg[0]].x - r.o.x) * d.inv_min.x;
+        F tmax = (b[1 - d.neg[0]].x - r.o.x) * d.inv_max.x;
+        F tymin = (b[d.neg[1]].y - r.o.y) * d.inv_min.y;
+        F tymax = (b[1 - d.neg[1]].y - r.o.y) * d.inv_max.y;
+        if ((tmin > tymax) || (tymin > tmax)) return false;
+        if (tymin > tmin) tmin = tymin;
+        if (tymax < tmax) tmax = tymax;
+        F tzmin = (b[d.neg[2]].z - r.o.z) * d.inv_min.z;
+        F tzmax = (b[1 - d.neg[2]].z - r.o.z) * d.inv_max.z;
+        if ((tmin > tzmax) || (tzmin > tmax)) return false;
+        if (tzmin > tmin) tmin = tzmin;
+        if (tzmax < tmax) tmax = tzmax;
+        return (tmin < r.maxT) && (tmax > r.minT);
+    }
+
+    // scene.bvh.cpp:405-448 (recursive; right child first if the ray points negative on the split axis)
+    bool visit(int ni, Trav& d) const {
+        const Node& nd = nodes[ni];
+        if (!box(nd.bound, d)) return false;
+        bool any = false;
+        if (nd.leaf) {
+            for (int i = nd.begin; i < nd.end; i++) {
+                F t, u, v;
+                if (ta[idx[i]].intersect(d.ray, d.ray.minT, d.ray.maxT, u, v, t)) {
+                    d.ray.maxT = t;
+                    d.hit.tri = ta[idx[i]].tri; d.hit.b0 = u; d.hit.b1 = v;
+                    any = true;
+                }
+            }
+        } else if (d.neg[nd.axis]) {
+            any |= visit(nd.right, d);
+            any |= visit(nd.left, d);
+        } else {
+            any |= visit(nd.left, d);
+            any |= visit(nd.right, d);
+        }
+        return any;
+    }
+
+    // scene.bvh.cpp:390-403 and BVHTraversalData (:124-150)
+    bool intersect(Ray<F>& ray, TriHit<F>& hit) const override {
+        if (nodes.empty()) return false;
+        Trav d{ray, {0, 0, 0}, V3<F>(), V3<F>(), TriHit<F>()};
+        for (int a = 0; a < 3; a++) {
+            F da = ray.d[a];
+            d.inv_min[a] = da == F(0) ? F(0) : F(F(1) / da);
+            d.inv_max[a] = da == F(0) ? K<F>::inf() : F(F(1) / da);
+            d.neg[a] = da < F(0);
+        }
+        if (visit(0, d)) { hit = d.hit; return true; }
+        return false;
+    }
+    const char* name() const override { return "bvh"; }
+};
+
+// ---------------------------------------------------------------------------------
+// QBVH (FP32 only) -- src/liblightmetrica/scene.qbvh.cpp:36-960
+//   * 4-wide node, SoA child bounds, child word: bit31 leaf | (quads-1)<<27 | offset.
+//   * leaves hold <= 64 triangles as quads of 4 (SSE mode, :441-443).
+//   * build: 12-bin SAH on the longest centroid axis, cost n1 A1 + n2 A2 (:697-774),
+//     partition by centroid <= split (:776-790), two binary levels per 4-wide node
+//     (:523-591), quads padded by duplicating the last triangle (:593-695).
+//   * traversal: 64-entry stack, children pushed 0..3 and popped LIFO (:837-960),
+//     4-wide slab test (:269-311), 4-wide Moeller-Trumbore accepting t in
+//     (minT, maxT) strictly, lowest lane first on ties (:98-150).
+
+struct QBVHAccel : Accel<float> {
+    struct alignas(16) Node {
+        __m128 b[2][3];  // b[0][axis] = min of 4 children, b[1][axis] = max
+        int32_t child[4];
+        Node() {
+            for (int a = 0; a < 3; a++) {
+                b[0][a] = _mm_set1_ps(std::numeric_limits<float>::infinity());
+                b[1][a] = _mm_set1_ps(-std::numeric_limits<float>::infinity());
+            }
+            for (int i = 0; i < 4; i++) child[i] = kEmpty;
+        }
+        void set_bound(int ci, const AABB<float>& bb) {
+            for (int a = 0; a < 3; a++) {
+                reinterpret_cast<float*>(&b[0][a])[ci] = bb.mn[a];
+                reinterpret_cast<float*>(&b[1][a])[ci] = bb.mx[a];
+            }
+        }
+    };
+    struct alignas(16) Quad {
+        __m128 ox, oy, oz, e1x, e1y, e1z, e2x, e2y, e2z;
+        uint32_t tri[4];
+    };
+    static constexpr int32_t kEmpty = int32_t(0xffffffffu);
+    static constexpr unsigned kMaxInLeaf = 64;
+
+    std::vector<Node> nodes;
+    std::vector<Quad> quads;
+    std::vector<uint32_t> idx;
+    std::vector<AABB<float>> tb;
+    std::vector<V3<float>> tc;
+    const TriSoup<float>& soup;
+    mutable std::atomic<long long> stack_overflows{0};
+    long long oversize_leaves = 0;
+
+    static int32_t encode_leaf(unsigned quads_n, unsigned offset) {
+        if (quads_n == 0) return kEmpty;
+        uint32_t w = 0x80000000u | (((quads_n - 1) & 0xfu) << 27) | (offset & 0x07ffffffu);
+        return int32_t(w);
+    }
+
+    explicit QBVHAccel(const TriSoup<float>& s) : soup(s) {
+        size_t n = s.size();
+        idx.resize(n); tb.resize(n); tc.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            idx[i] = uint32_t(i);
+            AABB<float> b(s.p1[i], s.p2[i]);
+            b = b.uni(s.p3[i]);
+            tb[i] = b;
+            tc[i] = (b.mn + b.mx) * 0.5f;
+        }
+        nodes.reserve(n / 16 + 16);
+        build(0, unsigned(n), -1, 0, 0);
+        if (nodes.empty()) nodes.push_back(Node());
+        post_build(0);
+    }
+
+    void create_leaf(unsigned begin, unsigned end, int parent, int ci, const AABB<float>& bound) {
+        if (parent < 0) { nodes.push_back(Node()); parent = 0; }
+        nodes[parent].set_bound(ci, bound);
+        if (end - begin > kMaxInLeaf) oversize_leaves++;
+        // Stores the number of quads; the offset is the first index into idx (:792-820).
+        nodes[parent].child[ci] = encode_leaf((end - begin + 3) / 4, begin);
+    }
+    unsigned create_inner(int parent, int ci, const AABB<float>& bound) {
+        unsigned created = unsigned(nodes.size());
+        nodes.push_back(Node());
+        if (parent >= 0) {
+            nodes[parent].child[ci] = int32_t(created);
+            nodes[parent].set_bound(ci, bound);
+        }
+        return created;
+    }
+
+    // :697-774
+    bool split_axis_and_position(unsigned begin, unsigned end, int& axis, float& pos) const {
+        AABB<float> cb;
+        for (unsigned i = begin; i < end; i++) cb = cb.uni(tc[idx[i]]);
+        axis = cb.longest_axis();
+        if (cb.mn[axis] == cb.mx[axis]) return false;
+        const int nb = 12;
+        const float k0 = cb.mn[axis];
+        const float k1 = float(nb) / (cb.mx[axis] - k0);
+        AABB<float> bb[nb];
+        int bc[nb] = {0};
+        for (unsigned i = begin; i < end; i++) {
+            unsigned t = idx[i];
+            int b = std::max(0, std::min(nb - 1, int(k1 * (tc[t][axis] - k0))));
+            bc[b]++;
+            bb[b] = bb[b].uni(tb[t]);
+        }
+        float costs[nb - 1];
+        for (int i = 0; i < nb - 1; i++) {
+            AABB<float> b1, b2;
+            int c1 = 0, c2 = 0;
+            for (int j = 0; j <= i; j++) { b1 = b1.uni(bb[j]); c1 += bc[j]; }
+            for (int j = i + 1; j < nb; j++) { b2 = b2.uni(bb[j]); c2 += bc[j]; }
+            costs[i] = float(c1) * b1.surface_area() + float(c2) * b2.surface_area();
+        }
+        int best = 0;
+        double min_cost = costs[0];
+        for (int i = 1; i < nb - 1; i++)
+            if (min_cost > costs[i]) { min_cost = costs[i]; best = i; }
+        pos = cb.mn[axis] + float(best + 1) * (cb.mx[axis] - cb.mn[axis]) / nb;
+        return true;
+    }
+
+    // :523-591
+    void build(unsigned begin, unsigned end, int parent, int ci, int depth) {
+        AABB<float> bound;
+        for (unsigned i = begin; i < end; i++) bound = bound.uni(tb[idx[i]]);
+        if (end - begin <= kMaxInLeaf) { create_leaf(begin, end, parent, ci, bound); return; }
+        int axis;
+        float pos;
+        if (!split_axis_and_position(begin, end, axis, pos)) { create_leaf(begin, end, parent, ci, bound); return; }
+        unsigned mid = begin;
+        for (unsigned i = begin; i < end; i++) {  // :776-790
+            unsigned t = idx[i];
+            if (tc[t][axis] <= pos) { idx[i] = idx[mid]; idx[mid] = t; mid++; }
+        }
+        if (mid == begin || mid == end) { create_leaf(begin, end, parent, ci, bound); return; }  // guard: see README
+        unsigned current;
+        int left, right;
+        if (depth % 2 == 1) {
+            current = unsigned(parent); left = ci; right = ci + 1;
+        } else {
+            current = create_inner(parent, ci, bound); left = 0; right = 2;
+        }
+        build(begin, mid, int(current), left, depth + 1);
+        build(mid, end, int(current), right, depth + 1);
+    }
+
+    // :593-695 -- quads in DFS order; a quad may include triangles that follow the
+    // leaf in idx (:625-626); the tail is padded by duplicating the last valid triangle.
+    void post_build(unsigned ni) {
+        for (int i = 0; i < 4; i++) {
+            int32_t cd = nodes[ni].child[i];
+            if (cd < 0) {
+                if (cd == kEmpty) continue;
+                unsigned size = unsigned(((cd >> 27) & 0xf) + 1);
+                unsigned offset = unsigned(cd) & 0x07ffffffu;
+                unsigned qoff = unsigned(quads.size());
+                for (unsigned j = 0; j < size; j++) {
+                    Quad q;
+                    std::memset(&q, 0, sizeof(q));
+                    float o[3][4], e1[3][4], e2[3][4];
+                    int endk = 0;
+                    V3<float> P1[4], P2[4], P3[4];
+                    for (int k = 0; k < 4; k++) {
+                        unsigned ti = offset + 4 * j + unsigned(k);
+                        if (ti < idx.size()) {
+                            endk = k;
+                            unsigned tri = idx[ti];
+                            q.tri[k] = tri;
+                            P1[k] = soup.p1[tri]; P2[k] = soup.p2[tri]; P3[k] = soup.p3[tri];
+                        }
+                    }
+                    for (int k = endk + 1; k < 4; k++) { P1[k] = P1[endk]; P2[k] = P2[endk]; P3[k] = P3[endk]; q.tri[k] = q.tri[endk]; }
+                    for (int k = 0; k < 4; k++) {
+                        for (int a = 0; a < 3; a++) {
+                            o[a][k] = P1[k][a];
+                            e1[a][k] = P2[k][a] - P1[k][a];
+                            e2[a][k] = P3[k][a] - P1[k][a];
+                        }
+                    }
+                    q.ox = _mm_loadu_ps(o[0]); q.oy = _mm_loadu_ps(o[1]); q.oz = _mm_loadu_ps(o[2]);
+                    q.e1x = _mm_loadu_ps(e1[0]); q.e1y = _mm_loadu_ps(e1[1]); q.e1z = _mm_loadu_ps(e1[2]);
+                    q.e2x = _mm_loadu_ps(e2[0]); q.e2y = _mm_loadu_ps(e2[1]); q.e2z = _mm_loadu_ps(e2[2]);
+                    quads.push_back(q);
+                }
+                nodes[ni].child[i] = encode_leaf(size, qoff);
+            } else {
+                post_build(unsigned(cd));
+            }
+        }
+    }
+
+    struct Ray4 { __m128 ox, oy, oz, dx, dy, dz, minT, maxT; };
+
+    // :98-150
+    static bool quad_intersect(const Quad& q, Ray4& r4, Ray<float>& ray, float& b0, float& b1, unsigned& lane) {
+        const __m128 zero = _mm_setzero_ps();
+        const __m128 one = _mm_set1_ps(1.0f);
+        const __m128 s1x = _mm_sub_ps(_mm_mul_ps(r4.dy, q.e2z), _mm_mul_ps(r4.dz, q.e2y));
+        const __m128 s1y = _mm_sub_ps(_mm_mul_ps(r4.dz, q.e2x), _mm_mul_ps(r4.dx, q.e2z));
+        const __m128 s1z = _mm_sub_ps(_mm_mul_ps(r4.dx, q.e2y), _mm_mul_ps(r4.dy, q.e2x));
+        const __m128 div = _mm_add_ps(_mm_mul_ps(s1x, q.e1x), _mm_add_ps(_mm_mul_ps(s1y, q.e1y), _mm_mul_ps(s1z, q.e1z)));
+        const __m128 zmask = _mm_cmpeq_ps(zero, div);
+        const __m128 sdiv = _mm_add_ps(div, _mm_and_ps(one, zmask));
+        __m128 hit = _mm_cmpneq_ps(div, zero);
+        const __m128 dx = _mm_sub_ps(r4.ox, q.ox);
+        const __m128 dy = _mm_sub_ps(r4.oy, q.oy);
+        const __m128 dz = _mm_sub_ps(r4.oz, q.oz);
+        const __m128 u = _mm_div_ps(_mm_add_ps(_mm_mul_ps(dx, s1x), _mm_add_ps(_mm_mul_ps(dy, s1y), _mm_mul_ps(dz, s1z))), sdiv);
+        hit = _mm_and_ps(hit, _mm_cmpge_ps(u, zero));
+        const __m128 s2x = _mm_sub_ps(_mm_mul_ps(dy, q.e1z), _mm_mul_ps(dz, q.e1y));
+        const __m128 s2y = _mm_sub_ps(_mm_mul_ps(dz, q.e1x), _mm_mul_ps(dx, q.e1z));
+        const __m128 s2z = _mm_sub_ps(_mm_mul_ps(dx, q.e1y), _mm_mul_ps(dy, q.e1x));
+        const __m128 v = _mm_div_ps(_mm_add_ps(_mm_mul_ps(r4.dx, s2x), _mm_add_ps(_mm_mul_ps(r4.dy, s2y), _mm_mul_ps(r4.dz, s2z))), sdiv);
+        const __m128 w = _mm_sub_ps(one, _mm_add_ps(u, v));
+        hit = _mm_and_ps(hit, _mm_and_ps(_mm_cmpge_ps(v, zero), _mm_cmpge_ps(w, zero)));
+        const __m128 t = _mm_div_ps(_mm_add_ps(_mm_mul_ps(q.e2x, s2x), _mm_add_ps(_mm_mul_ps(q.e2y, s2y), _mm_mul_ps(q.e2z, s2z))), sdiv);
+        hit = _mm_and_ps(hit, _mm_and_ps(_mm_cmpgt_ps(t, r4.minT), _mm_cmplt_ps(t, r4.maxT)));
+        alignas(16) int hm[4];
+        alignas(16) float tv[4], uv[4], vv[4];
+        _mm_store_ps(reinterpret_cast<float*>(hm), hit);
+        _mm_store_ps(tv, t); _mm_store_ps(uv, u); _mm_store_ps(vv, v);
+        unsigned best = 4;
+        for (unsigned i = 0; i < 4; i++) {
+            if (hm[i] && tv[i] < ray.maxT) { best = i; ray.maxT = tv[i]; }
+        }
+        if (best == 4) return false;
+        r4.maxT = _mm_set1_ps(ray.maxT);
+        lane = best;
+        b0 = uv[best];
+        b1 = vv[best];
+        return true;
+    }
+
+    bool intersect(Ray<float>& ray, TriHit<float>& hit) const override {
+        Ray4 r4;
+        r4.ox = _mm_set1_ps(ray.o.x); r4.oy = _mm_set1_ps(ray.o.y); r4.oz = _mm_set1_ps(ray.o.z);
+        r4.dx = _mm_set1_ps(ray.d.x); r4.dy = _mm_set1_ps(ray.d.y); r4.dz = _mm_set1_ps(ray.d.z);
+        r4.minT = _mm_set1_ps(ray.minT); r4.maxT = _mm_set1_ps(ray.maxT);
+        __m128 inv_min[3], inv_max[3];
+        int sign[3];
+        for (int a = 0; a < 3; a++) {  // :860-866
+            float da = ray.d[a];
+            inv_min[a] = _mm_set1_ps(da == 0.0f ? K<float>::eps() : 1.0f / da);
+            inv_max[a] = _mm_set1_ps(da == 0.0f ? K<float>::inf() : 1.0f / da);
+            sign[a] = da < 0.0f;
+        }
+        const int kStack = 64;
+        int32_t stack[kStack + 4];
+        int sp = 0;
+        stack[0] = 0;
+        bool found = false;
+        unsigned hq = 0, hl = 0;
+        float hb0 = 0, hb1 = 0;
+        while (sp >= 0) {
+            int32_t data = stack[sp--];
+            if (data < 0) {
+                if (data == kEmpty) continue;
+                unsigned size = unsigned(((data >> 27) & 0xf) + 1);
+                unsigned off = unsigned(data) & 0x07ffffffu;
+                for (unsigned i = off; i < off + size; i++) {
+                    float b0, b1;
+                    unsigned lane;
+                    if (quad_intersect(quads[i], r4, ray, b0, b1, lane)) {
+                        hq = i; hl = lane; hb0 = b0; hb1 = b1; found = true;
+                    }
+                }
+            } else {
+                const Node& nd = nodes[size_t(data)];
+                __m128 mn = r4.minT, mx = r4.maxT;
+                const __m128* o[3] = {&r4.ox, &r4.oy, &r4.oz};
+                for (int a = 0; a < 3; a++) {  // :294-309
+                    mn = _mm_max_ps(mn, _mm_mul_ps(_mm_sub_ps(nd.b[sign[a]][a], *o[a]), inv_min[a]));
+                    mx = _mm_min_ps(mx, _mm_mul_ps(_mm_sub_ps(nd.b[1 - sign[a]][a], *o[a]), inv_max[a]));
+                }
+                int mask = _mm_movemask_ps(_mm_cmpge_ps(mx, mn));
+                for (int c = 0; c < 4; c++) {
+                    if (mask & (1 << c)) {
+                        if (sp + 1 >= kStack) { stack_overflows++; continue; }  // the reference has no check
+                        stack[++sp] = nd.child[c];
+                    }
+                }
+            }
+        }
+        if (!found) return false;
+        hit.tri = quads[hq].tri[hl];
+        hit.b0 = hb0;
+        hit.b1 = hb1;
+        return true;
+    }
+    const char* name() const override { return "qbvh"; }
+};
+
+// ---------------------------------------------------------------------------------
+// Scene data (flattened primitives, world space)
+
+enum MaterialType : uint32_t { kDiffuse = 0, kMirror = 1, kDiffuseMirror = 2 };
+enum LightType : uint32_t { kArea = 0, kEnvConst = 1 };
+
+// GeneralizedBSDFType bits -- include/lightmetrica/generalizedbsdf.h:34-72
+enum BsdfBits : int {
+    kDiffuseReflection = 1 << 0,
+    kSpecularReflection = 1 << 2,
+    kNonDeltaLightDirection = 1 << 7,
+    kNonDeltaEyeDirection = 1 << 9,
+    kSpecular = (1 << 2) | (1 << 3),
+    kNonDelta = ((1 << 10) - 1) & ~((1 << 2) | (1 << 3) | (1 << 6) | (1 << 8)),
+};
+
+// SurfaceGeometry -- include/lightmetrica/surfacegeometry.h:40-62
+template <class F> struct Geom {
+    V3<F> p, gn, sn, ss, st;
+    bool degenerated = false;
+    // OrthonormalBasis(sn, ss, st) -- math.linalgebra.inl:26-33
+    void tangent_space() {
+        st = std::abs(sn.x) > std::abs(sn.y) ? normalize(V3<F>(sn.z, F(0), -sn.x))
+                                             : normalize(V3<F>(F(0), sn.z, -sn.y));
+        ss = normalize(cross(st, sn));
+    }
+    // worldToShading = transpose(Mat3(ss, st, sn)); shadingToWorld = Mat3(ss, st, sn)
+    V3<F> to_local(const V3<F>& w) const { return V3<F>(dot(ss, w), dot(st, w), dot(sn, w)); }
+    V3<F> to_world(const V3<F>& l) const {
+        return V3<F>(ss.x * l.x + st.x * l.y + sn.x * l.z,
+                     ss.y * l.x + st.y * l.y + sn.y * l.z,
+                     ss.z * l.x + st.z * l.y + sn.z * l.z);
+    }
+};
+
+struct MaterialDesc { uint32_t type; float R[3]; };
+struct LightDesc { uint32_t type; float Le[3]; };
+struct CameraDesc {
+    float view[16];  // column-major 4x4 world->camera (the camera primitive's transform)
+    float fovy_deg;
+    uint32_t width, height;
+};
+
+template <class F> struct AreaLight {
+    V3<F> Le;
+    std::vector<uint32_t> tris;
+    std::vector<F> cdf;  // light.area.cpp:109-144
+    F area = 0, inv_area = 0;
+};
+
+template <class F> struct Camera {
+    M4<F> view, inv_view, proj, inv_proj;
+    V3<F> position;
+    F inv_A = 0;
+    int width = 1, height = 1;
+
+    // perspectivecamera.cpp:97-172 and :174-184
+    void setup(const M4<F>& v, F fovy, F aspect) {
+        view = v;
+        inv_view = inverse(view);
+        position = (inv_view * V4<F>(F(0), F(0), F(0), F(1))).xyz();
+        proj = perspective(fovy, aspect, F(1), F(1000));
+        inv_proj = inverse(proj);
+        V4<F> c1 = inv_proj * V4<F>(V3<F>(F(-1), F(-1), F(0)), F(1));
+        V4<F> c2 = inv_proj * V4<F>(V3<F>(F(1), F(1), F(0)), F(1));
+        V3<F> p1 = c1.xyz() / c1.w;
+        V3<F> p2 = c2.xyz() / c1.w;  // the reference divides both by camP1_4.w (:163)
+        p1 = V3<F>(p1.x / p1.z, p1.y / p1.z, p1.z / p1.z);
+        p2 = V3<F>(p2.x / p2.z, p2.y / p2.z, p2.z / p2.z);
+        F A = (p2.x - p1.x) * (p2.y - p1.y);
+        inv_A = F(1) / A;
+    }
+    // perspectivecamera.cpp:325-353
+    F importance(F cos_theta) const {
+        if (cos_theta <= F(0)) return F(0);
+        if (std::abs(F(cos_theta - F(1))) < K<F>::eps_large()) cos_theta = F(1);
+        F ic = F(1) / cos_theta;
+        return inv_A * ic * ic * ic;
+    }
+    // perspectivecamera.cpp:216-240 -- returns the sampled direction; estimate is 1.
+    V3<F> sample_direction(const V2<F>& raster, F& pdf) const {
+        V3<F> ndc(raster.x * F(2) - F(1), raster.y * F(2) - F(1), F(0));
+        V4<F> d4 = inv_proj * V4<F>(ndc, F(1));
+        V3<F> d3 = normalize(d4.xyz() / d4.w);
+        V3<F> wo = normalize((inv_view * V4<F>(d3, F(0))).xyz());
+        pdf = importance(-d3.z);
+        return wo;
+    }
+    // perspectivecamera.cpp:304-323
+    bool ray_to_raster(const V3<F>& p, const V3<F>& d, V2<F>& raster) const {
+        V4<F> rc = view * V4<F>(p + d, F(1));
+        V4<F> rn = proj * rc;
+        V3<F> n3 = rn.xyz() / rn.w;
+        raster = V2<F>((n3.x + F(1)) / F(2), (n3.y + F(1)) / F(2));
+        return !(raster.x < F(0) || raster.x > F(1) || raster.y < F(0) || raster.y > F(1));
+    }
+    // perspectivecamera.cpp:276-302 (importance for a direction leaving the camera)
+    F evaluate_direction(const V3<F>& p, const V3<F>& wo) const {
+        V4<F> rc = view * V4<F>(p + wo, F(1));
+        V4<F> rn = proj * rc;
+        V3<F> n3 = rn.xyz() / rn.w;
+        V2<F> r((n3.x + F(1)) / F(2), (n3.y + F(1)) / F(2));
+        if (r.x < F(0) || r.x > F(1) || r.y < F(0) || r.y > F(1)) return F(0);
+        return importance(-normalize(rc.xyz()).z);
+    }
+};
+
+template <class F> struct Scene {
+    TriSoup<F> soup;
+    std::vector<V3<F>> n1, n2, n3;  // per-triangle vertex normals (world)
+    std::vector<uint32_t> mat_of_tri;
+    std::vector<int32_t> light_of_tri;
+    std::vector<MaterialDesc> materials;
+    std::vector<LightDesc> light_desc;
+    std::vector<AreaLight<F>> area;      // indexed like light_desc (unused for env)
+    int env = -1;                        // index of the env light, or -1
+    V3<F> env_center;
+    F env_radius = 0, env_area = 0, env_inv_area = 0;
+    Camera<F> cam;
+    std::unique_ptr<Accel<F>> accel;
+    AABB<F> tri_aabb;
+
+    int num_lights() const { return int(light_desc.size()); }
+};
+
+// Hit record after reconstruction (Intersection, include/lightmetrica/intersection.h:40-60)
+template <class F> struct Isect {
+    Geom<F> g;
+    int light = -1;     // light index or -1
+    int mat = -1;       // material index; -2 = black diffuse of the env sphere
+    uint32_t tri = 0;
+    bool env_hit = false;
+};
+
+// ---------------------------------------------------------------------------------
+// BSDFs -- diffuse (src/liblightmetrica/diffuse.cpp:117-220), mirror (mirror.cpp:68-215),
+// diffuse/mirror mix (plugin/plugin.diffusemirror/diffusemirror.cpp:29-288).
+// The shading-normal correction is bsdf.h:57-83.
+
+template <class F> inline F shading_correction(bool le, const Geom<F>& g, const V3<F>& lwi, const V3<F>& lwo,
+                                               const V3<F>& wi, const V3<F>& wo) {
+    F wi_ng = dot(wi, g.gn), wo_ng = dot(wo, g.gn);
+    F wi_ns = lwi.z, wo_ns = lwo.z;
+    if (wi_ng * wi_ns <= F(0) || wo_ng * wo_ns <= F(0)) return F(0);
+    if (le) return wi_ns * wo_ng / (wo_ns * wi_ng);
+    return F(1);
+}
+
+template <class F> struct BsdfSample {
+    V3<F> wo;
+    F pdf = 0;
+    int type = 0;  // sampled GeneralizedBSDFType bit
+};
+
+template <class F> inline int bsdf_types(int mat, const std::vector<MaterialDesc>& mats) {
+    if (mat < 0) return kDiffuseReflection;
+    switch (mats[size_t(mat)].type) {
+        case kMirror: return kSpecularReflection;
+        case kDiffuseMirror: return kDiffuseReflection | kSpecularReflection;
+        default: return kDiffuseReflection;
+    }
+}
+
+template <class F> inline V3<F> mat_R(int mat, const std::vector<MaterialDesc>& mats) {
+    if (mat < 0) return V3<F>();
+    const MaterialDesc& m = mats[size_t(mat)];
+    return V3<F>(F(m.R[0]), F(m.R[1]), F(m.R[2]));
+}
+
+// SampleAndEstimateDirection (EL transport; query type covers all BSDF components).
+template <class F> V3<F> bsdf_sample_estimate(int mat, const std::vector<MaterialDesc>& mats, const Geom<F>& g,
+                                              const V3<F>& wi, const V2<F>& u, F ucomp, BsdfSample<F>& s) {
+    V3<F> lwi = g.to_local(wi);
+    uint32_t type = mat < 0 ? kDiffuse : mats[size_t(mat)].type;
+    V3<F> R = mat_R<F>(mat, mats);
+    if (type == kDiffuse) {  // diffuse.cpp:133-157
+        if (lwi.z <= F(0)) return V3<F>();
+        V3<F> lwo = cosine_hemisphere(u);
+        s.wo = g.to_world(lwo);
+        s.type = kDiffuseReflection;
+        s.pdf = kInvPi<F>();
+        F sf = shading_correction(false, g, lwi, lwo, wi, s.wo);
+        if (sf == F(0)) return V3<F>();
+        return R * sf;
+    }
+    if (type == kMirror) {  // mirror.cpp:84-107
+        if (lwi.z <= F(0)) return V3<F>();
+        V3<F> lwo = reflect_zup(lwi);
+        s.wo = g.to_world(lwo);
+        s.type = kSpecularReflection;
+        s.pdf = F(1) / lwo.z;
+        F sf = shading_correction(false, g, lwi, lwo, wi, s.wo);
+        if (sf == F(0)) return V3<F>();
+        return R * sf;
+    }
+    // diffusemirror.cpp:84-125 (ComponentProb = DiffuseWeight = SpecularWeight = 0.5)
+    const F cp = F(0.5), dw = F(0.5), sw = F(0.5);
+    F cos_i = lwi.z;
+    if (cos_i <= F(0)) return V3<F>();
+    if (ucomp < cp) {
+        V3<F> lwo = cosine_hemisphere(u);
+        s.wo = g.to_world(lwo);
+        s.type = kDiffuseReflection;
+        s.pdf = kInvPi<F>() * cp;
+        F sf = shading_correction(false, g, lwi, lwo, wi, s.wo);
+        if (sf == F(0)) return V3<F>();
+        return R * sf * dw / cp;
+    }
+    V3<F> lwo = reflect_zup(lwi);
+    s.wo = g.to_world(lwo);
+    s.type = kSpecularReflection;
+    s.pdf = cp / std::abs(cos_i);
+    F sf = shading_correction(false, g, lwi, lwo, wi, s.wo);
+    if (sf == F(0)) return V3<F>();
+    return R * sf * sw / cp;
+}
+
+// EvaluateDirection with query type NonDelta (EL) -- diffuse.cpp:192-208,
+// mirror.cpp:141-180 (NonDelta excludes the specular bit -> 0), diffusemirror.cpp:188-236.
+template <class F> V3<F> bsdf_eval_nondelta(int mat, const std::vector<MaterialDesc>& mats, const Geom<F>& g,
+                                            const V3<F>& wi, const V3<F>& wo) {
+    V3<F> lwi = g.to_local(wi), lwo = g.to_local(wo);
+    uint32_t type = mat < 0 ? kDiffuse : mats[size_t(mat)].type;
+    if (type == kMirror) return V3<F>();
+    if (lwi.z <= F(0) || lwo.z <= F(0)) return V3<F>();
+    F sf = shading_correction(false, g, lwi, lwo, wi, wo);
+    if (sf == F(0)) return V3<F>();
+    V3<F> R = mat_R<F>(mat, mats);
+    if (type == kDiffuse) return R * kInvPi<F>() * sf;
+    return R * kInvPi<F>() * sf * F(0.5);
+}
+
+// EvaluateDirectionPDF with query type NonDelta -- diffuse.cpp:210-220, diffusemirror.cpp:238-274
+template <class F> F bsdf_pdf_nondelta(int mat, const std::vector<MaterialDesc>& mats, const Geom<F>& g,
+                                       const V3<F>& wi, const V3<F>& wo) {
+    V3<F> lwi = g.to_local(wi), lwo = g.to_local(wo);
+    uint32_t type = mat < 0 ? kDiffuse : mats[size_t(mat)].type;
+    if (type == kMirror) return F(0);
+    if (lwi.z <= F(0) || lwo.z <= F(0)) return F(0);
+    if (type == kDiffuse) return kInvPi<F>();
+    return kInvPi<F>() * F(0.5);
+}
+
+// ---------------------------------------------------------------------------------
+// Scene queries
+
+// Scene::StoreIntersectionFromBarycentricCoords -- src/liblightmetrica/scene.cpp:101-149
+template <class F> void store_intersection(const Scene<F>& sc, const Ray<F>& ray, const TriHit<F>& h, Isect<F>& is) {
+    is.tri = h.tri;
+    is.mat = int(sc.mat_of_tri[h.tri]);
+    is.light = sc.light_of_tri[h.tri];
+    is.env_hit = false;
+    is.g.p = ray.o + ray.d * ray.maxT;
+    const V3<F>& p1 = sc.soup.p1[h.tri];
+    const V3<F>& p2 = sc.soup.p2[h.tri];
+    const V3<F>& p3 = sc.soup.p3[h.tri];
+    is.g.gn = normalize(cross(p2 - p1, p3 - p1));
+    is.g.sn = normalize(sc.n1[h.tri] * (F(1) - h.b0 - h.b1) + sc.n2[h.tri] * h.b0 + sc.n3[h.tri] * h.b1);
+    is.g.degenerated = false;
+    is.g.tangent_space();
+}
+
+// SphereEmitterShape::Intersect / StoreIntersection -- src/liblightmetrica/emittershape.sphere.cpp:117-176
+template <class F> bool env_intersect(const Scene<F>& sc, Ray<F>& ray, Isect<F>& is) {
+    if (sc.env < 0) return false;
+    V3<F> o = ray.o - sc.env_center;
+    V3<F> d = ray.d;
+    F a = length2(d);
+    F b = F(2) * dot(o, d);
+    F c = length2(o) - sc.env_radius * sc.env_radius;
+    F det = b * b - F(4) * a * c;
+    if (det < F(0)) return false;
+    F e = std::sqrt(det);
+    F denom = F(2) * a;
+    F t0 = (-b - e) / denom, t1 = (-b + e) / denom;
+    if (t0 > ray.maxT || t1 < ray.minT) return false;
+    F t = t0;
+    if (t < ray.minT) {
+        t = t1;
+        if (t > ray.maxT) return false;
+    }
+    ray.maxT = t;  // Primitives::IntersectEmitterShapes updates the ray (primitives.cpp:214-238)
+    is.g.p = ray.o + ray.d * ray.maxT;
+    is.g.gn = is.g.sn = normalize(sc.env_center - is.g.p);
+    is.g.tangent_space();
+    is.g.degenerated = false;
+    is.light = sc.env;
+    is.mat = -2;  // black diffuse (emittershape.sphere.cpp:95-101)
+    is.env_hit = true;
+    return true;
+}
+
+// Scene::Intersect -- src/liblightmetrica/scene.cpp:67-72
+template <class F> bool scene_intersect(const Scene<F>& sc, Ray<F>& ray, Isect<F>& is) {
+    TriHit<F> h;
+    if (sc.accel->intersect(ray, h)) {
+        store_intersection(sc, ray, h, is);
+        return true;
+    }
+    return env_intersect(sc, ray, is);
+}
+
+// RenderUtils::Visible -- src/liblightmetrica/renderutils.cpp:58-70 (a closest-hit query)
+template <class F> bool visible(const Scene<F>& sc, const V3<F>& p1, const V3<F>& p2) {
+    V3<F> v = p2 - p1;
+    F len = length(v);
+    Ray<F> r;
+    r.d = v / len;
+    r.o = p1;
+    r.minT = K<F>::eps();
+    r.maxT = len * (F(1) - K<F>::eps());
+    Isect<F> is;
+    return !scene_intersect(sc, r, is);
+}
+
+// RenderUtils::GeneralizedGeometryTerm -- renderutils.cpp:29-56
+template <class F> F geometry_term(const Geom<F>& g1, const Geom<F>& g2) {
+    V3<F> v = g2.p - g1.p;
+    F l2 = length2(v);
+    F l = std::sqrt(l2);
+    v /= l;
+    F num(1);
+    if (!g1.degenerated) num *= std::abs(dot(g1.sn, v));
+    if (!g2.degenerated) num *= std::abs(dot(g2.sn, -v));
+    if (std::abs(l) < K<F>::eps()) return F(0);
+    return num / l2;
+}
+
+// Scene::SampleLightSelection (Vec2 overload) -- scene.cpp:79-86
+template <class F> int sample_light_selection(const Scene<F>& sc, V2<F>& u, F& sel_pdf) {
+    int nl = sc.num_lights();
+    int li = std::min(int(u.x * F(nl)), nl - 1);
+    u.x = u.x * F(nl) - F(li);
+    sel_pdf = F(1) / F(nl);
+    return li;
+}
+
+// Light::SamplePosition -- AreaLight light.area.cpp:146-179; env light.env.const.cpp:196-204
+template <class F> void light_sample_position(const Scene<F>& sc, int li, const V2<F>& u, Geom<F>& g, F& pdf) {
+    if (sc.light_desc[size_t(li)].type == kEnvConst) {
+        V3<F> d = uniform_sphere(u);
+        g.degenerated = false;
+        g.p = sc.env_center + d * sc.env_radius;
+        g.gn = g.sn = -d;
+        g.tangent_space();
+        pdf = sc.env_inv_area;
+        return;
+    }
+    const AreaLight<F>& L = sc.area[size_t(li)];
+    V2<F> ps = u;
+    int n = int(L.cdf.size());
+    int index = int(std::upper_bound(L.cdf.begin(), L.cdf.end(), ps.y) - L.cdf.begin()) - 1;
+    index = std::max(0, std::min(index, n - 2));
+    ps.y = (ps.y - L.cdf[size_t(index)]) / (L.cdf[size_t(index) + 1] - L.cdf[size_t(index)]);
+    uint32_t t = L.tris[size_t(index)];
+    const V3<F>& p1 = sc.soup.p1[t];
+    const V3<F>& p2 = sc.soup.p2[t];
+    const V3<F>& p3 = sc.soup.p3[t];
+    V2<F> b = uniform_triangle(ps);
+    g.p = p1 * (F(1) - b.x - b.y) + p2 * b.x + p3 * b.y;
+    g.gn = normalize(cross(p2 - p1, p3 - p1));
+    g.sn = g.gn;
+    g.tangent_space();
+    g.degenerated = false;
+    pdf = L.inv_area;
+}
+
+template <class F> V3<F> light_Le(const Scene<F>& sc, int li) {
+    const LightDesc& d = sc.light_desc[size_t(li)];
+    return V3<F>(F(d.Le[0]), F(d.Le[1]), F(d.Le[2]));
+}
+// EvaluatePosition = Le * pi (light.area.cpp:233-236, light.env.const.cpp:206-209)
+template <class F> V3<F> light_eval_position(const Scene<F>& sc, int li) { return light_Le(sc, li) * kPi<F>(); }
+// EvaluateDirection (LE) = 1/pi on the front side (light.area.cpp:243-252, light.env.const.cpp:174-183)
+template <class F> F light_eval_direction(const Geom<F>& g, const V3<F>& wo) {
+    V3<F> l = g.to_local(wo);
+    return l.z <= F(0) ? F(0) : kInvPi<F>();
+}
+template <class F> F light_position_pdf(const Scene<F>& sc, int li) {
+    return sc.light_desc[size_t(li)].type == kEnvConst ? sc.env_inv_area : sc.area[size_t(li)].inv_area;
+}
+
+// ---------------------------------------------------------------------------------
+// Film -- HDRBitmapFilm::AccumulateContribution, src/liblightmetrica/hdrfilm.cpp:151-171
+
+template <class F> struct Film {
+    int w = 0, h = 0;
+    std::vector<F> data;
+    long long dropped = 0;
+    void alloc(int W, int H) { w = W; h = H; data.assign(size_t(W) * H * 3, F(0)); }
+    void accumulate(const V2<F>& r, const V3<F>& c) {
+        if (r.x < F(0) || F(1) < r.x || r.y < F(0) || F(1) < r.y) { dropped++; return; }
+        int px = std::max(0, std::min(w - 1, int(F(r.x * F(w)))));
+        int py = std::max(0, std::min(h - 1, int(F(r.y * F(h)))));
+        size_t i = size_t(py) * w + px;
+        data[3 * i] += c.x;
+        data[3 * i + 1] += c.y;
+        data[3 * i + 2] += c.z;
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Estimators.  `rays` counts closest-hit queries actually traced (including the
+// shadow queries of Visible()).
+
+struct RenderParams {
+    int max_path_vertices = -1;  // pathtrace.cpp:126
+    int rr_depth = 1;            // pathtrace.cpp:125
+};
+
+// PathtraceRenderer_RenderProcess::ProcessSingleSample -- src/liblightmetrica/pathtrace.cpp:165-266
+template <class F> void pt_sample(const Scene<F>& sc, const RenderParams& rp, Sampler<F>& smp, Film<F>& film,
+                                  long long& rays) {
+    V2<F> raster = smp.next2();
+    (void)smp.next2();  // Camera::SamplePosition draws a Vec2 (:173); pinhole ignores it
+    F pdf_cam;
+    Ray<F> ray;
+    ray.o = sc.cam.position;
+    ray.d = sc.cam.sample_direction(raster, pdf_cam);
+    ray.minT = F(0);
+    ray.maxT = K<F>::inf();
+    V3<F> T(F(1));  // We estimate = 1 (perspectivecamera.cpp:239)
+    V3<F> L;
+    int n = 1;
+    while (true) {
+        if (rp.max_path_vertices != -1 && n >= rp.max_path_vertices) break;
+        Isect<F> is;
+        rays++;
+        if (!scene_intersect(sc, ray, is)) break;
+        if (is.light >= 0) {  // :210-220
+            F LeD = light_eval_direction(is.g, -ray.d);
+            L += T * V3<F>(LeD) * light_eval_position(sc, is.light);
+        }
+        V2<F> u = smp.next2();
+        F uc = smp.next();
+        BsdfSample<F> bs;
+        V3<F> fs = bsdf_sample_estimate(is.mat, sc.materials, is.g, -ray.d, u, uc, bs);
+        if (is_zero(fs)) break;
+        T *= fs;
+        ray.d = bs.wo;
+        ray.o = is.g.p;
+        ray.minT = K<F>::eps();
+        ray.maxT = K<F>::inf();
+        if (rp.rr_depth != -1 && n >= rp.rr_depth) {  // :250-260
+            F p = std::min(F(0.5), luminance(T));
+            if (smp.next() > p) break;
+            T /= p;
+        }
+        n++;
+    }
+    film.accumulate(raster, L);
+}
+
+// DirectPathtraceRenderer (pt.direct) -- src/liblightmetrica/pathtrace.direct.cpp:170-338, and
+// MISPathtraceRenderer (pt.mis) -- src/liblightmetrica/pathtrace.mis.cpp:167-369.
+// The two share their structure; `mis` switches the weights of :240-258 and :338-357.
+template <class F> void pt_nee_sample(const Scene<F>& sc, const RenderParams& rp, Sampler<F>& smp, Film<F>& film,
+                                      long long& rays, bool mis) {
+    (void)smp.next2();  // Camera::SamplePosition (:175)
+    Geom<F> cur;
+    cur.p = sc.cam.position;
+    cur.degenerated = true;
+    V3<F> T(F(1));  // positionalWe / pdfPE = 1 / 1
+    V3<F> cur_wi;
+    int cur_mat = -3;  // -3 = the camera
+    int n = 1;
+    V2<F> raster;
+    auto types_of = [&](int m) { return m == -3 ? int(kNonDeltaEyeDirection) : bsdf_types<F>(m, sc.materials); };
+    while (true) {
+        if (rp.max_path_vertices != -1 && n >= rp.max_path_vertices) break;
+        if (types_of(cur_mat) & kNonDelta) {
+            Geom<F> gl;
+            F pdf_pl;
+            V2<F> lu = smp.next2();
+            F sel_pdf;
+            int li = sample_light_selection(sc, lu, sel_pdf);
+            light_sample_position(sc, li, lu, gl, pdf_pl);
+            pdf_pl *= sel_pdf;
+            V3<F> ppl = normalize(gl.p - cur.p);
+            rays++;
+            if (visible(sc, cur.p, gl.p)) {
+                bool vis = true;
+                if (n == 1) vis = sc.cam.ray_to_raster(cur.p, ppl, raster);
+                if (vis) {
+                    V3<F> fsE;
+                    if (cur_mat == -3) fsE = V3<F>(sc.cam.evaluate_direction(cur.p, ppl));
+                    else fsE = bsdf_eval_nondelta(cur_mat, sc.materials, cur, cur_wi, ppl);
+                    F fsL = light_eval_direction(gl, -ppl);
+                    F G = geometry_term(cur, gl);
+                    V3<F> LeP = light_eval_position(sc, li);
+                    if (!mis) {
+                        film.accumulate(raster, T * fsE * G * fsL * LeP / pdf_pl);
+                    } else if (G != F(0)) {
+                        F pdf_dl = pdf_pl / G;
+                        F pdf_bsdf;
+                        if (cur_mat == -3) pdf_bsdf = sc.cam.evaluate_direction(cur.p, ppl);  // EvaluateDirectionPDF (:355-383)
+                        else pdf_bsdf = bsdf_pdf_nondelta(cur_mat, sc.materials, cur, cur_wi, ppl);
+                        F w = pdf_dl / (pdf_dl + pdf_bsdf);
+                        film.accumulate(raster, T * fsE * G * fsL * LeP * w / pdf_pl);
+                    }
+                }
+            }
+        }
+        if (rp.rr_depth != -1 && n >= rp.rr_depth) {
+            F p = std::min(F(0.5), luminance(T));
+            if (smp.next() > p) break;
+            T /= p;
+        }
+        V2<F> u = smp.next2();
+        F uc = smp.next();
+        BsdfSample<F> bs;
+        V3<F> fs;
+        if (cur_mat == -3) {
+            F pdfw;
+            bs.wo = sc.cam.sample_direction(u, pdfw);
+            bs.pdf = pdfw;
+            bs.type = kNonDeltaEyeDirection;
+            fs = V3<F>(F(1));
+        } else {
+            fs = bsdf_sample_estimate(cur_mat, sc.materials, cur, cur_wi, u, uc, bs);
+        }
+        if (is_zero(fs)) break;
+        T *= fs;
+        if (n == 1) {
+            if (!sc.cam.ray_to_raster(cur.p, bs.wo, raster)) break;
+        }
+        Ray<F> ray;
+        ray.d = bs.wo;
+        ray.o = cur.p;
+        ray.minT = K<F>::eps();
+        ray.maxT = K<F>::inf();
+        Isect<F> is;
+        rays++;
+        if (!scene_intersect(sc, ray, is)) break;
+        if (is.light >= 0) {
+            F LeD = light_eval_direction(is.g, -ray.d);
+            V3<F> LeP = light_eval_position(sc, is.light);
+            if (bs.type & kSpecular) {
+                film.accumulate(raster, T * LeD * LeP);
+            } else if (mis) {
+                F G = geometry_term(cur, is.g);
+                F pdf_dl = G == F(0) ? F(0) : (F(1) / F(sc.num_lights())) * light_position_pdf(sc, is.light) / G;
+                F w = bs.pdf / (bs.pdf + pdf_dl);
+                film.accumulate(raster, T * LeD * LeP * w);
+            }
+        }
+        cur = is.g;
+        cur_wi = -ray.d;
+        cur_mat = is.mat;
+        n++;
+    }
+}
+
+enum RendererKind : int { kPT = 0, kPTDirect = 1, kPTMIS = 2 };
+
+template <class F> void process_sample(int kind, const Scene<F>& sc, const RenderParams& rp, Sampler<F>& smp,
+                                       Film<F>& film, long long& rays) {
+    if (kind == kPT) pt_sample(sc, rp, smp, film, rays);
+    else pt_nee_sample(sc, rp, smp, film, rays, kind == kPTMIS);
+}
+
+}  // namespace lmref

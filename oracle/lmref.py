@@ -1,0 +1,268 @@
+"""ctypes binding of the CPU restatement (oracle/liblmref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
+The product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "build" / "liblmref.so"
+
+ACCELS = {"naive": 0, "bvh": 1, "qbvh": 2}
+KINDS = {"pt": 0, "pt.direct": 1, "pt.mis": 2}
+RNGS = {"philox": 0, "mt": 1}
+
+
+class _Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("R", C.c_float * 3)]
+
+
+class _Light(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("Le", C.c_float * 3)]
+
+
+class _Camera(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("fovy_deg", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    vp, fp, dp, u32p = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_uint32)
+    lib.lmref_scene_new.restype = vp
+    lib.lmref_scene_new.argtypes = [C.c_int, C.c_int, fp, C.c_uint32, fp, u32p, C.c_uint32, u32p,
+                                    C.POINTER(C.c_int32), C.POINTER(_Material), C.c_uint32, C.POINTER(_Light),
+                                    C.c_uint32, C.POINTER(_Camera)]
+    lib.lmref_scene_free.argtypes = [vp]
+    lib.lmref_set_approx_rsqrt.argtypes = [C.c_int]
+    lib.lmref_accel_name.argtypes = [vp]
+    lib.lmref_accel_name.restype = C.c_char_p
+    lib.lmref_qbvh_stack_overflows.argtypes = [vp]
+    lib.lmref_qbvh_stack_overflows.restype = C.c_longlong
+    lib.lmref_trace_closest.argtypes = [vp, fp, fp, fp, fp, C.c_uint64, C.c_int, dp, dp, dp, C.POINTER(C.c_int32)]
+    lib.lmref_intersect_geom.argtypes = [vp, fp, fp, C.c_float, dp]
+    lib.lmref_render.argtypes = [vp, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
+                                 C.c_longlong, C.c_uint64, dp, dp]
+    lib.lmref_philox.argtypes = [u32p, u32p, u32p]
+    lib.lmref_philox_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p]
+    lib.lmref_concentric_disk.argtypes = [C.c_int, dp, dp]
+    lib.lmref_cosine_hemisphere.argtypes = [C.c_int, dp, dp]
+    lib.lmref_uniform_sphere.argtypes = [C.c_int, dp, dp]
+    lib.lmref_look_at.argtypes = [dp, dp, dp, dp]
+    lib.lmref_perspective.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
+    lib.lmref_camera_direction.argtypes = [C.c_int, fp, C.c_float, C.c_float, dp, dp]
+    lib.lmref_camera_raster.argtypes = [C.c_int, fp, C.c_float, C.c_float, dp, dp, dp]
+    lib.lmref_camera_raster.restype = C.c_int
+    lib.lmref_stub_random_mesh.argtypes = [dp]
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """One flattened scene in the CPU restatement (FP32 or FP64)."""
+
+    def __init__(self, scene, fp64: bool = True, accel: str = "bvh", approx_rsqrt: bool = False):
+        self.lib = load()
+        self.lib.lmref_set_approx_rsqrt(1 if approx_rsqrt else 0)
+        self.scene = scene
+        self.fp64 = fp64
+        P, Nn = _f(scene.positions), _f(scene.normals)
+        F = np.ascontiguousarray(scene.faces, dtype=np.uint32)
+        nf = F.shape[0]
+        if scene.material_ids is not None:
+            M = np.ascontiguousarray(scene.material_ids, dtype=np.uint32)
+            L = np.ascontiguousarray(scene.light_ids, dtype=np.int32)
+            mats = (_Material * len(scene.materials))()
+            for i, m in enumerate(scene.materials):
+                mats[i].type = int(m["type"])
+                mats[i].R[:] = [float(x) for x in m["R"]]
+            nl = len(scene.lights)
+            lights = (_Light * max(1, nl))()
+            for i in range(nl):
+                lights[i].type = int(scene.lights[i]["type"])
+                lights[i].Le[:] = [float(x) for x in scene.lights[i]["Le"]]
+            nmat = len(scene.materials)
+        else:
+            M = np.zeros(nf, np.uint32)
+            L = np.full(nf, -1, np.int32)
+            mats = (_Material * 1)()
+            lights = (_Light * 1)()
+            nmat, nl = 1, 0
+        cam = _Camera()
+        if scene.camera is not None:
+            cam.view[:] = [float(x) for x in np.asarray(scene.camera.view, np.float32)]
+            cam.fovy_deg = float(scene.camera.fovy)
+            cam.width, cam.height = scene.camera.width, scene.camera.height
+        else:
+            cam.view[:] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]
+            cam.fovy_deg, cam.width, cam.height = 90.0, 1, 1
+        self._keep = (P, Nn, F, M, L, mats, lights, cam)
+        self.h = self.lib.lmref_scene_new(
+            int(fp64), ACCELS[accel], P.ctypes.data_as(C.POINTER(C.c_float)), P.shape[0],
+            Nn.ctypes.data_as(C.POINTER(C.c_float)), F.ctypes.data_as(C.POINTER(C.c_uint32)), nf,
+            M.ctypes.data_as(C.POINTER(C.c_uint32)), L.ctypes.data_as(C.POINTER(C.c_int32)), mats, nmat, lights, nl,
+            C.byref(cam))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.lmref_scene_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def accel(self) -> str:
+        return self.lib.lmref_accel_name(self.h).decode()
+
+    def stack_overflows(self) -> int:
+        return int(self.lib.lmref_qbvh_stack_overflows(self.h))
+
+    def trace_closest(self, rays, threads: int = 0):
+        n = len(rays)
+        o, d = _f(rays.o).reshape(-1), _f(rays.d).reshape(-1)
+        tmin, tmax = _f(rays.tmin), _f(rays.tmax)
+        t, b0, b1 = np.empty(n), np.empty(n), np.empty(n)
+        tri = np.empty(n, np.int32)
+        fp = C.POINTER(C.c_float)
+        dp = C.POINTER(C.c_double)
+        self.lib.lmref_trace_closest(self.h, o.ctypes.data_as(fp), d.ctypes.data_as(fp), tmin.ctypes.data_as(fp),
+                                     tmax.ctypes.data_as(fp), n, threads, t.ctypes.data_as(dp), b0.ctypes.data_as(dp),
+                                     b1.ctypes.data_as(dp), tri.ctypes.data_as(C.POINTER(C.c_int32)))
+        return t, b0, b1, tri
+
+    def intersect_geom(self, o, d, tmin: float = 0.0):
+        out = np.zeros(15)
+        o, d = _f(o), _f(d)
+        fp = C.POINTER(C.c_float)
+        ok = self.lib.lmref_intersect_geom(self.h, o.ctypes.data_as(fp), d.ctypes.data_as(fp), float(tmin),
+                                           out.ctypes.data_as(C.POINTER(C.c_double)))
+        if not ok:
+            return None
+        return {"p": out[0:3], "gn": out[3:6], "sn": out[6:9], "ss": out[9:12], "st": out[12:15]}
+
+    def render(self, renderer: str = "pt", num_samples=None, max_path_vertices=None, rr_depth=None, seed=None,
+               rng: str = "philox", threads: int = 0, samples_per_block: int = 100, sample_offset: int = 0):
+        rc = self.scene.render
+        W, H = self.scene.camera.width, self.scene.camera.height
+        film = np.zeros(W * H * 3)
+        stats = np.zeros(5)
+        dp = C.POINTER(C.c_double)
+        st = self.lib.lmref_render(
+            self.h, KINDS[renderer], int(num_samples if num_samples is not None else rc.num_samples),
+            int(max_path_vertices if max_path_vertices is not None else rc.max_path_vertices),
+            int(rr_depth if rr_depth is not None else rc.rr_depth), int(seed if seed is not None else rc.seed),
+            RNGS[rng], threads, samples_per_block, sample_offset, film.ctypes.data_as(dp), stats.ctypes.data_as(dp))
+        if st != 0:
+            raise RuntimeError(f"lmref_render failed ({st})")
+        return film.reshape(H, W, 3), {"rays": int(stats[0]), "seconds": float(stats[1]),
+                                       "dropped": int(stats[2]), "samples": int(stats[3]),
+                                       "threads": int(stats[4])}
+
+
+def philox(key, ctr):
+    lib = load()
+    k = (C.c_uint32 * 2)(*key)
+    c = (C.c_uint32 * 4)(*ctr)
+    o = (C.c_uint32 * 4)()
+    lib.lmref_philox(k, c, o)
+    return list(o)
+
+
+def philox_stream(seed: int, sample: int, n: int) -> np.ndarray:
+    lib = load()
+    out = np.empty(n, np.uint32)
+    lib.lmref_philox_stream(seed, sample, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def _vec_fn(name, fp64, u, nout):
+    lib = load()
+    u = _d(u)
+    out = np.zeros(nout)
+    getattr(lib, name)(int(fp64), u.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def concentric_disk(u, fp64=True):
+    return _vec_fn("lmref_concentric_disk", fp64, u, 2)
+
+
+def cosine_hemisphere(u, fp64=True):
+    return _vec_fn("lmref_cosine_hemisphere", fp64, u, 3)
+
+
+def uniform_sphere(u, fp64=True):
+    return _vec_fn("lmref_uniform_sphere", fp64, u, 3)
+
+
+def look_at(eye, center, up) -> np.ndarray:
+    lib = load()
+    out = np.zeros(16)
+    dp = C.POINTER(C.c_double)
+    e, c, u = _d(eye), _d(center), _d(up)
+    lib.lmref_look_at(e.ctypes.data_as(dp), c.ctypes.data_as(dp), u.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    return out
+
+
+def perspective(fovy, aspect, zn, zf) -> np.ndarray:
+    lib = load()
+    out = np.zeros(16)
+    lib.lmref_perspective(fovy, aspect, zn, zf, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def camera_direction(view16, fovy, aspect, raster, fp64=True) -> np.ndarray:
+    lib = load()
+    v = _f(view16)
+    r = _d(raster)
+    out = np.zeros(3)
+    dp = C.POINTER(C.c_double)
+    lib.lmref_camera_direction(int(fp64), v.ctypes.data_as(C.POINTER(C.c_float)), fovy, aspect, r.ctypes.data_as(dp),
+                               out.ctypes.data_as(dp))
+    return out
+
+
+def camera_raster(view16, fovy, aspect, p, d, fp64=True):
+    lib = load()
+    v = _f(view16)
+    p, d = _d(p), _d(d)
+    out = np.zeros(2)
+    dp = C.POINTER(C.c_double)
+    ok = lib.lmref_camera_raster(int(fp64), v.ctypes.data_as(C.POINTER(C.c_float)), fovy, aspect,
+                                 p.ctypes.data_as(dp), d.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    return bool(ok), out
+
+
+def stub_random_values() -> np.ndarray:
+    lib = load()
+    out = np.zeros(9000)
+    lib.lmref_stub_random_mesh(out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out

@@ -1,0 +1,318 @@
+// lmref C API -- TEST INFRASTRUCTURE ONLY (see lmref.hpp).
+//
+// A flat C interface so the Python tests and bench.py's cpu_baseline leg can drive
+// the CPU restatement through ctypes.  Nothing in the product links this library.
+#include "lmref.hpp"
+
+#include <chrono>
+#include <omp.h>
+
+namespace lmref {
+bool g_approx_rsqrt = true;
+}
+
+using namespace lmref;
+
+namespace {
+
+struct SceneHandle {
+    int fp64 = 0;
+    std::unique_ptr<Scene<float>> s32;
+    std::unique_ptr<Scene<double>> s64;
+};
+
+template <class F>
+std::unique_ptr<Scene<F>> make_scene(int accel, const float* pos, uint32_t nv, const float* nrm, const uint32_t* faces,
+                                     uint32_t nf, const uint32_t* mat, const int32_t* light_of_tri,
+                                     const MaterialDesc* mats, uint32_t nmat, const LightDesc* lights, uint32_t nl,
+                                     const CameraDesc* cam) {
+    (void)nv;
+    std::unique_ptr<Scene<F>> sc(new Scene<F>());
+    sc->soup.p1.resize(nf); sc->soup.p2.resize(nf); sc->soup.p3.resize(nf);
+    sc->n1.resize(nf); sc->n2.resize(nf); sc->n3.resize(nf);
+    sc->mat_of_tri.assign(mat, mat + nf);
+    sc->light_of_tri.assign(light_of_tri, light_of_tri + nf);
+    sc->materials.assign(mats, mats + nmat);
+    sc->light_desc.assign(lights, lights + nl);
+    auto P = [&](uint32_t v) { return V3<F>(F(pos[3 * v]), F(pos[3 * v + 1]), F(pos[3 * v + 2])); };
+    auto N = [&](uint32_t v) { return V3<F>(F(nrm[3 * v]), F(nrm[3 * v + 1]), F(nrm[3 * v + 2])); };
+    for (uint32_t t = 0; t < nf; t++) {
+        uint32_t a = faces[3 * t], b = faces[3 * t + 1], c = faces[3 * t + 2];
+        sc->soup.p1[t] = P(a); sc->soup.p2[t] = P(b); sc->soup.p3[t] = P(c);
+        sc->n1[t] = N(a); sc->n2[t] = N(b); sc->n3[t] = N(c);
+        AABB<F> bb(sc->soup.p1[t], sc->soup.p2[t]);
+        sc->tri_aabb = sc->tri_aabb.uni(bb.uni(sc->soup.p3[t]));
+    }
+    // Area lights: AreaLight::RegisterPrimitives (light.area.cpp:109-144), triangles in scene order.
+    sc->area.resize(nl);
+    for (uint32_t l = 0; l < nl; l++) {
+        if (lights[l].type == kEnvConst) { sc->env = int(l); continue; }
+        AreaLight<F>& A = sc->area[l];
+        A.Le = V3<F>(F(lights[l].Le[0]), F(lights[l].Le[1]), F(lights[l].Le[2]));
+        A.cdf.push_back(F(0));
+        for (uint32_t t = 0; t < nf; t++) {
+            if (light_of_tri[t] != int32_t(l)) continue;
+            A.tris.push_back(t);
+            const V3<F>&p1 = sc->soup.p1[t], &p2 = sc->soup.p2[t], &p3 = sc->soup.p3[t];
+            F ar = length(cross(p2 - p1, p3 - p1)) / F(2);
+            A.cdf.push_back(A.cdf.back() + ar);
+        }
+        A.area = A.cdf.back();
+        A.inv_area = F(1) / A.area;
+        for (auto& v : A.cdf) v *= A.inv_area;
+    }
+    // Camera
+    M4<F> view;
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) view.c[c][r] = F(cam->view[c * 4 + r]);
+    sc->cam.width = int(cam->width);
+    sc->cam.height = int(cam->height);
+    sc->cam.setup(view, F(cam->fovy_deg), F(cam->width) / F(cam->height));
+    // Env light bounding sphere: ConstantEnvironmentLight::PostConfigure (light.env.const.cpp:88-98)
+    // over Scene::GetAABB = triangles + camera position (scene.cpp:58-65).
+    if (sc->env >= 0) {
+        AABB<F> bb = sc->tri_aabb.uni(sc->cam.position);
+        sc->env_center = (bb.mx + bb.mn) / F(2);
+        sc->env_radius = length(sc->env_center - bb.mx);
+        sc->env_area = F(4) * kPi<F>() * sc->env_radius * sc->env_radius;
+        sc->env_inv_area = F(1) / sc->env_area;
+    }
+    return sc;
+}
+
+template <class F> Accel<F>* make_accel(int accel, const TriSoup<F>& s);
+template <> Accel<float>* make_accel<float>(int accel, const TriSoup<float>& s) {
+    if (accel == 0) return new NaiveAccel<float>(s);
+    if (accel == 1) return new BVHAccel<float>(s);
+    return new QBVHAccel(s);
+}
+template <> Accel<double>* make_accel<double>(int accel, const TriSoup<double>& s) {
+    if (accel == 0) return new NaiveAccel<double>(s);
+    return new BVHAccel<double>(s);  // the reference has no QBVH in double mode
+}
+
+template <class F>
+void trace_all(const Scene<F>& sc, const float* o, const float* d, const float* tmin, const float* tmax, uint64_t n,
+               int nthreads, double* t_out, double* b0_out, double* b1_out, int32_t* tri_out) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (long long i = 0; i < (long long)n; i++) {
+        Ray<F> r;
+        r.o = V3<F>(F(o[3 * i]), F(o[3 * i + 1]), F(o[3 * i + 2]));
+        r.d = V3<F>(F(d[3 * i]), F(d[3 * i + 1]), F(d[3 * i + 2]));
+        r.minT = F(tmin[i]);
+        r.maxT = tmax[i] >= FLT_MAX ? K<F>::inf() : F(tmax[i]);
+        TriHit<F> h;
+        if (sc.accel->intersect(r, h)) {
+            t_out[i] = double(r.maxT); b0_out[i] = double(h.b0); b1_out[i] = double(h.b1); tri_out[i] = int32_t(h.tri);
+        } else {
+            t_out[i] = -1.0; b0_out[i] = 0; b1_out[i] = 0; tri_out[i] = -1;
+        }
+    }
+}
+
+// MTRenderProcessScheduler::Render -- src/liblightmetrica/sched.mt.cpp:93-266 (TerminationMode::Samples).
+template <class F>
+int render_mt(const Scene<F>& sc, int kind, const RenderParams& rp, uint64_t num_samples, uint64_t seed, int rng,
+              int nthreads, long long spb, uint64_t sample_offset, double* film_out, double* stats) {
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    omp_set_num_threads(nthreads);
+    const int W = sc.cam.width, H = sc.cam.height;
+    std::vector<Film<F>> films(static_cast<size_t>(nthreads));
+    std::vector<std::unique_ptr<RngSource>> srcs(static_cast<size_t>(nthreads));
+    std::vector<long long> rays(static_cast<size_t>(nthreads), 0);
+    std::mt19937 master(static_cast<uint32_t>(seed));  // master sampler; per-thread seeds = master.NextUInt() (pathtrace.cpp:158-159)
+    for (int i = 0; i < nthreads; i++) {
+        films[size_t(i)].alloc(W, H);
+        if (rng == 0) srcs[size_t(i)].reset(new PhiloxSource(seed));
+        else srcs[size_t(i)].reset(new MTSource(uint32_t(master())));
+    }
+    const long long N = (long long)num_samples;
+    const long long blocks = (N + spb) / spb;  // sched.mt.cpp:100
+    std::atomic<long long> processed(0);
+    auto t0 = std::chrono::high_resolution_clock::now();
+#pragma omp parallel for
+    for (long long b = 0; b < blocks; b++) {
+        int tid = omp_get_thread_num();
+        Sampler<F> smp{srcs[size_t(tid)].get()};
+        long long sb = spb * b;
+        long long se = std::min(sb + spb, N);
+        if (se > sb) processed += se - sb;
+        for (long long s = sb; s < se; s++) {
+            srcs[size_t(tid)]->begin_sample(sample_offset + uint64_t(s));
+            process_sample(kind, sc, rp, smp, films[size_t(tid)], rays[size_t(tid)]);
+        }
+    }
+    auto t1 = std::chrono::high_resolution_clock::now();
+    // Serial reduce into the master film, then Rescale(W*H / processed) (sched.mt.cpp:250-256).
+    Film<F> master_film;
+    master_film.alloc(W, H);
+    long long dropped = 0, total_rays = 0;
+    for (int i = 0; i < nthreads; i++) {
+        for (size_t k = 0; k < master_film.data.size(); k++) master_film.data[k] += films[size_t(i)].data[k];
+        dropped += films[size_t(i)].dropped;
+        total_rays += rays[size_t(i)];
+    }
+    F weight = F(W * H) / F(processed.load());
+    for (size_t k = 0; k < master_film.data.size(); k++) film_out[k] = double(master_film.data[k] * weight);
+    if (stats) {
+        stats[0] = double(total_rays);
+        stats[1] = std::chrono::duration<double>(t1 - t0).count();
+        stats[2] = double(dropped);
+        stats[3] = double(processed.load());
+        stats[4] = double(nthreads);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* lmref_scene_new(int fp64, int accel, const float* pos, uint32_t nv, const float* nrm, const uint32_t* faces,
+                      uint32_t nf, const uint32_t* mat, const int32_t* light_of_tri, const MaterialDesc* mats,
+                      uint32_t nmat, const LightDesc* lights, uint32_t nl, const CameraDesc* cam) {
+    SceneHandle* h = new SceneHandle();
+    h->fp64 = fp64;
+    if (fp64) {
+        h->s64 = make_scene<double>(accel, pos, nv, nrm, faces, nf, mat, light_of_tri, mats, nmat, lights, nl, cam);
+        h->s64->accel.reset(make_accel<double>(accel, h->s64->soup));
+    } else {
+        h->s32 = make_scene<float>(accel, pos, nv, nrm, faces, nf, mat, light_of_tri, mats, nmat, lights, nl, cam);
+        h->s32->accel.reset(make_accel<float>(accel, h->s32->soup));
+    }
+    return h;
+}
+
+void lmref_scene_free(void* p) { delete static_cast<SceneHandle*>(p); }
+
+void lmref_set_approx_rsqrt(int on) { g_approx_rsqrt = on != 0; }
+
+const char* lmref_accel_name(void* p) {
+    SceneHandle* h = static_cast<SceneHandle*>(p);
+    return h->fp64 ? h->s64->accel->name() : h->s32->accel->name();
+}
+
+long long lmref_qbvh_stack_overflows(void* p) {
+    SceneHandle* h = static_cast<SceneHandle*>(p);
+    if (h->fp64) return 0;
+    auto* q = dynamic_cast<QBVHAccel*>(h->s32->accel.get());
+    return q ? q->stack_overflows.load() : 0;
+}
+
+int lmref_trace_closest(void* p, const float* o, const float* d, const float* tmin, const float* tmax, uint64_t n,
+                        int nthreads, double* t_out, double* b0_out, double* b1_out, int32_t* tri_out) {
+    SceneHandle* h = static_cast<SceneHandle*>(p);
+    if (h->fp64) trace_all(*h->s64, o, d, tmin, tmax, n, nthreads, t_out, b0_out, b1_out, tri_out);
+    else trace_all(*h->s32, o, d, tmin, tmax, n, nthreads, t_out, b0_out, b1_out, tri_out);
+    return 0;
+}
+
+// Full hit reconstruction for one ray (KAT tests: p, gn, sn, ss, st).  out[15].
+int lmref_intersect_geom(void* p, const float* o, const float* d, float tmin, double* out) {
+    SceneHandle* h = static_cast<SceneHandle*>(p);
+    auto run = [&](auto& sc) {
+        using F = std::decay_t<decltype(sc.soup.p1[0].x)>;
+        Ray<F> r;
+        r.o = V3<F>(F(o[0]), F(o[1]), F(o[2]));
+        r.d = V3<F>(F(d[0]), F(d[1]), F(d[2]));
+        r.minT = F(tmin);
+        r.maxT = K<F>::inf();
+        Isect<F> is;
+        if (!scene_intersect(sc, r, is)) return 0;
+        const V3<F>* v[5] = {&is.g.p, &is.g.gn, &is.g.sn, &is.g.ss, &is.g.st};
+        for (int i = 0; i < 5; i++)
+            for (int a = 0; a < 3; a++) out[3 * i + a] = double((*v[i])[a]);
+        return 1;
+    };
+    return h->fp64 ? run(*h->s64) : run(*h->s32);
+}
+
+int lmref_render(void* p, int kind, uint64_t num_samples, int max_path_vertices, int rr_depth, uint64_t seed, int rng,
+                 int nthreads, long long samples_per_block, uint64_t sample_offset, double* film_out, double* stats) {
+    SceneHandle* h = static_cast<SceneHandle*>(p);
+    RenderParams rp;
+    rp.max_path_vertices = max_path_vertices;
+    rp.rr_depth = rr_depth;
+    if (samples_per_block <= 0) return -1;
+    if (h->fp64) return render_mt(*h->s64, kind, rp, num_samples, seed, rng, nthreads, samples_per_block, sample_offset, film_out, stats);
+    return render_mt(*h->s32, kind, rp, num_samples, seed, rng, nthreads, samples_per_block, sample_offset, film_out, stats);
+}
+
+// ---- known-answer helpers ---------------------------------------------------------
+
+void lmref_philox(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]) { Philox4x32::block(key, ctr, out); }
+
+// u32 draws of the per-sample stream (seed, sample), dims [0, n)
+void lmref_philox_stream(uint64_t seed, uint64_t sample, uint32_t n, uint32_t* out) {
+    PhiloxSource s(seed);
+    s.begin_sample(sample);
+    for (uint32_t i = 0; i < n; i++) out[i] = s.next_u32();
+}
+
+void lmref_concentric_disk(int fp64, const double* u, double* out) {
+    if (fp64) { auto r = concentric_disk(V2<double>(u[0], u[1])); out[0] = r.x; out[1] = r.y; }
+    else { auto r = concentric_disk(V2<float>(float(u[0]), float(u[1]))); out[0] = r.x; out[1] = r.y; }
+}
+void lmref_cosine_hemisphere(int fp64, const double* u, double* out) {
+    if (fp64) { auto r = cosine_hemisphere(V2<double>(u[0], u[1])); out[0] = r.x; out[1] = r.y; out[2] = r.z; }
+    else { auto r = cosine_hemisphere(V2<float>(float(u[0]), float(u[1]))); out[0] = r.x; out[1] = r.y; out[2] = r.z; }
+}
+void lmref_uniform_sphere(int fp64, const double* u, double* out) {
+    if (fp64) { auto r = uniform_sphere(V2<double>(u[0], u[1])); out[0] = r.x; out[1] = r.y; out[2] = r.z; }
+    else { auto r = uniform_sphere(V2<float>(float(u[0]), float(u[1]))); out[0] = r.x; out[1] = r.y; out[2] = r.z; }
+}
+
+void lmref_look_at(const double* eye, const double* center, const double* up, double* out16) {
+    M4<double> m = look_at(V3<double>(eye[0], eye[1], eye[2]), V3<double>(center[0], center[1], center[2]),
+                           V3<double>(up[0], up[1], up[2]));
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) out16[c * 4 + r] = m.c[c][r];
+}
+void lmref_perspective(double fovy, double aspect, double zn, double zf, double* out16) {
+    M4<double> m = perspective(fovy, aspect, zn, zf);
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) out16[c * 4 + r] = m.c[c][r];
+}
+
+// Camera KATs: sample a direction for a raster position / map a ray back to raster.
+void lmref_camera_direction(int fp64, const float* view16, float fovy, float aspect, const double* raster, double* out3) {
+    auto run = [&](auto zero) {
+        using F = decltype(zero);
+        Camera<F> c;
+        M4<F> v;
+        for (int col = 0; col < 4; col++)
+            for (int r = 0; r < 4; r++) v.c[col][r] = F(view16[col * 4 + r]);
+        c.setup(v, F(fovy), F(aspect));
+        F pdf;
+        V3<F> d = c.sample_direction(V2<F>(F(raster[0]), F(raster[1])), pdf);
+        out3[0] = d.x; out3[1] = d.y; out3[2] = d.z;
+    };
+    if (fp64) run(0.0); else run(0.0f);
+}
+int lmref_camera_raster(int fp64, const float* view16, float fovy, float aspect, const double* p, const double* d,
+                        double* out2) {
+    auto run = [&](auto zero) {
+        using F = decltype(zero);
+        Camera<F> c;
+        M4<F> v;
+        for (int col = 0; col < 4; col++)
+            for (int r = 0; r < 4; r++) v.c[col][r] = F(view16[col * 4 + r]);
+        c.setup(v, F(fovy), F(aspect));
+        V2<F> rp;
+        bool ok = c.ray_to_raster(V3<F>(F(p[0]), F(p[1]), F(p[2])), V3<F>(F(d[0]), F(d[1]), F(d[2])), rp);
+        out2[0] = rp.x; out2[1] = rp.y;
+        return ok ? 1 : 0;
+    };
+    return fp64 ? run(0.0) : run(0.0f);
+}
+
+// StubTriangleMesh_Random (include/lightmetrica.test/stub.trianglemesh.h:184-225):
+// std::mt19937(42) + uniform_real_distribution<double>, 1000 faces, 9 doubles per face.
+void lmref_stub_random_mesh(double* out9000) {
+    std::mt19937 gen(42);
+    std::uniform_real_distribution<double> dist;
+    for (int i = 0; i < 9000; i++) out9000[i] = dist(gen);
+}
+
+}  // extern "C"

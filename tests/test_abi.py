@@ -1,0 +1,56 @@
+"""The C-ABI library loads and exports every entry point include/lmgpu.h declares (no GPU needed)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "lmgpu.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:lmgpu_status|const char\*|int)\s+(lmgpu_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_abi():
+    names = declared_functions()
+    for must in ("lmgpu_create", "lmgpu_scene_upload", "lmgpu_bvh_build", "lmgpu_trace_closest", "lmgpu_trace_any",
+                 "lmgpu_pt_render", "lmgpu_last_error", "lmgpu_destroy"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from lightmetrica_amd import _native
+
+    lib = _native.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared_functions()) == set(_native.EXPORTS)
+
+
+def test_abi_version():
+    from lightmetrica_amd import _native
+
+    assert _native.load().lmgpu_abi_version() == 1
+
+
+def test_null_context_is_rejected_without_gpu():
+    from lightmetrica_amd import _native
+
+    lib = _native.load()
+    assert lib.lmgpu_destroy(None) == 1
+    assert lib.lmgpu_synchronize(None) == 1
+    assert lib.lmgpu_last_error(None) == b"null context"
+    out = ctypes.c_void_p()
+    # an invalid device ordinal is an argument error, never a crash
+    assert lib.lmgpu_create(-1, ctypes.byref(out)) == 1
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from lightmetrica_amd import _native
+
+    monkeypatch.setattr(_native, "_lib", None)
+    monkeypatch.setattr(_native, "LIB_PATH", tmp_path / "liblmgpu.so")
+    with pytest.raises(ImportError):
+        _native.load()

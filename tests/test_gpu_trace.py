@@ -1,0 +1,193 @@
+"""GPU traversal parity: HIP closest-hit / any-hit kernels vs the FP64 oracle (BVH + TriAccel).
+
+Bar: primitive ids identical except near ties
+(|t1 - t2| / t < 1e-6) and near edges (a barycentric coordinate within 1e-5 of an edge,
+where the reference's non-watertight TriAccel and the watertight GPU test may legitimately
+disagree); t, b0, b1 within 1e-5 relative (FP64 refinement of the winner).
+"""
+import numpy as np
+import pytest
+
+from lightmetrica_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+NEAR_TIE = 1e-6
+NEAR_EDGE = 1e-5
+REL_TOL = 1e-5
+
+
+def classify(t_g, b0_g, b1_g, p_g, t_o, b0_o, b1_o, p_o):
+    """Returns (#identical, #near-tie, #near-edge, #unexplained) over the rays."""
+    same = p_g == p_o
+    both = (p_g >= 0) & (p_o >= 0)
+    tie = ~same & both & (np.abs(t_g - t_o) <= NEAR_TIE * np.abs(t_o))
+    # near an edge of either candidate triangle (the oracle's or the GPU's)
+    def edge(b0, b1):
+        return (np.minimum(np.minimum(b0, b1), 1 - b0 - b1) < NEAR_EDGE)
+    near_edge = ~same & ~tie & ((edge(b0_o, b1_o) & (p_o >= 0)) | (edge(b0_g, b1_g) & (p_g >= 0)))
+    unexplained = ~same & ~tie & ~near_edge
+    return int(same.sum()), int(tie.sum()), int(near_edge.sum()), np.where(unexplained)[0]
+
+
+def rel_err(a, b, floor=1e-6):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def test_intersect_simple_kat(gpu):
+    """Intersect_Simple (test.scene.intersection.cpp:117-150): p = (x, y, 0)."""
+    gpu.upload(scenes.stub_simple_mesh())
+    gpu.build_bvh()
+    rays = scenes.grid_rays_simple()
+    t, b0, b1, prim = gpu.trace_closest(rays)
+    assert np.all(prim >= 0) and np.all(prim <= 1)
+    p = rays.o.astype(np.float64) + rays.d.astype(np.float64) * t[:, None]
+    xs = np.arange(1, 10) / 10
+    expect = np.array([[x, y, 0] for y in xs for x in xs])
+    np.testing.assert_allclose(p, expect, atol=1e-6)
+
+
+def test_intersect_simple2_kat(gpu):
+    """Intersect_Simple2 (test.scene.intersection.cpp:152-187): p = (x, y, -x)."""
+    gpu.upload(scenes.stub_simple2_mesh())
+    gpu.build_bvh()
+    rays = scenes.grid_rays_down()
+    t, b0, b1, prim = gpu.trace_closest(rays)
+    assert np.all(prim >= 0)
+    p = rays.o.astype(np.float64) + rays.d.astype(np.float64) * t[:, None]
+    expect = np.stack([rays.o[:, 0], rays.o[:, 1], -rays.o[:, 0]], axis=1)
+    np.testing.assert_allclose(p, expect, atol=1e-6)
+
+
+def test_consistency_random_mesh(gpu, oracle):
+    """Consistency (test.scene.intersection.cpp:190-266) against the FP64 naive oracle."""
+    sc = scenes.stub_random_mesh(oracle.stub_random_values())
+    gpu.upload(sc)
+    gpu.build_bvh()
+    rays = scenes.grid_rays_down()
+    g = gpu.trace_closest(rays)
+    o = oracle.Oracle(sc, fp64=True, accel="naive").trace_closest(rays)
+    same, tie, edge, bad = classify(*g, *o)
+    assert len(bad) == 0, bad
+    hit = (g[3] >= 0) & (g[3] == o[3])
+    assert rel_err(g[0][hit], o[0][hit]).max() < REL_TOL
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8])
+def test_traversal_parity_reduced(gpu, oracle, leaf):
+    """Reduced traversal microbench: 64k triangles x 256k rays vs the FP64 oracle."""
+    sc = scenes.traversal_scene(num_triangles=65536, seed=0xC2A)
+    rays = scenes.traversal_rays(262144, seed=0xC2B)
+    gpu.upload(sc)
+    info = gpu.build_bvh(max_leaf_size=leaf)
+    assert info.num_tri_records == 65536 and info.max_depth * 3 <= 96
+    t, b0, b1, prim = gpu.trace_closest(rays, refine_fp64=True)
+    o = oracle.Oracle(sc, fp64=True, accel="bvh")
+    to, b0o, b1o, po = o.trace_closest(rays)
+    same, tie, edge, bad = classify(t, b0, b1, prim, to, b0o, b1o, po)
+    print(f"leaf={leaf}: identical {same}, near-tie {tie}, near-edge {edge}, unexplained {len(bad)}", flush=True)
+    assert len(bad) == 0, [(i, prim[i], po[i], t[i], to[i]) for i in bad[:10]]
+    assert same >= len(rays) - 50
+    hit = (prim >= 0) & (prim == po)
+    assert hit.sum() > 10000
+    assert rel_err(t[hit], to[hit]).max() < REL_TOL
+    assert rel_err(b0[hit], b0o[hit]).max() < REL_TOL
+    assert rel_err(b1[hit], b1o[hit]).max() < REL_TOL
+
+
+def test_fp32_path_is_close_without_refinement(gpu, oracle):
+    sc = scenes.traversal_scene(num_triangles=65536, seed=0xC2A)
+    rays = scenes.traversal_rays(65536, seed=0xC2B)
+    gpu.upload(sc)
+    gpu.build_bvh()
+    t, b0, b1, prim = gpu.trace_closest(rays, refine_fp64=False)
+    to, b0o, b1o, po = oracle.Oracle(sc, fp64=True, accel="bvh").trace_closest(rays)
+    hit = (prim >= 0) & (prim == po)
+    # plain FP32 watertight: t to ~1e-6, barycentrics to ~1e-4 (cancellation on small triangles)
+    assert np.median(rel_err(t[hit], to[hit])) < 1e-6
+    assert np.abs(b0[hit] - b0o[hit]).max() < 1e-3
+
+
+def test_any_hit_matches_closest(gpu):
+    sc = scenes.traversal_scene(num_triangles=65536, seed=11)
+    rays = scenes.traversal_rays(131072 + 77, seed=12)  # ragged size
+    gpu.upload(sc)
+    gpu.build_bvh()
+    t, _, _, prim = gpu.trace_closest(rays, refine_fp64=False)
+    occ = gpu.trace_any(rays)
+    np.testing.assert_array_equal(occ, prim >= 0)
+    # finite tmax: occluded iff the closest hit is closer than tmax (away from ties at tmax)
+    tmax = np.full(len(rays), 0.3, np.float32)
+    r2 = scenes.Rays(rays.o, rays.d, rays.tmin, tmax)
+    occ2 = gpu.trace_any(r2)
+    ref = (prim >= 0) & (t < 0.3)
+    clear = np.abs(t - 0.3) > 1e-5
+    np.testing.assert_array_equal(occ2[clear], ref[clear])
+
+
+def test_tmin_excludes_near_hits(gpu):
+    gpu.upload(scenes.stub_simple_mesh())
+    gpu.build_bvh()
+    rays = scenes.grid_rays_down()
+    t, _, _, prim = gpu.trace_closest(rays)
+    assert np.allclose(t, 1.0)  # plane z = 0 first
+    r2 = scenes.Rays(rays.o, rays.d, np.full(len(rays), 1.5, np.float32), rays.tmax)
+    t2, _, _, prim2 = gpu.trace_closest(r2)
+    assert np.allclose(t2, 2.0) and np.all(prim2 >= 2)  # plane z = -1 next
+
+
+def test_empty_and_tiny_batches(gpu):
+    gpu.upload(scenes.stub_simple_mesh())
+    gpu.build_bvh()
+    rays = scenes.grid_rays_down()
+    t, b0, b1, prim = gpu.trace_closest(rays.slice(0, 0))
+    assert len(t) == 0
+    t, b0, b1, prim = gpu.trace_closest(rays.slice(0, 1))
+    assert prim[0] >= 0
+
+
+def test_all_misses(gpu):
+    gpu.upload(scenes.stub_simple_mesh())
+    gpu.build_bvh()
+    o = np.tile(np.array([[5, 5, 5]], np.float32), (100, 1))
+    d = np.tile(np.array([[0, 0, 1]], np.float32), (100, 1))
+    t, _, _, prim = gpu.trace_closest(scenes.Rays(o, d, np.zeros(100, np.float32),
+                                                  np.full(100, scenes.FLT_MAX, np.float32)))
+    assert np.all(prim == -1) and np.all(t == -1)
+
+
+def test_deterministic(gpu):
+    sc = scenes.traversal_scene(num_triangles=20000, seed=3)
+    rays = scenes.traversal_rays(50000, seed=4)
+    gpu.upload(sc)
+    gpu.build_bvh()
+    a = gpu.trace_closest(rays)
+    b = gpu.trace_closest(rays)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_full_size_properties(gpu, oracle):
+    """Full traversal workload (1M triangles, 16M rays): size-independent properties, plus
+    oracle parity on a strided subset."""
+    sc = scenes.traversal_scene()
+    rays = scenes.traversal_rays()
+    gpu.upload(sc)
+    info = gpu.build_bvh()
+    print(f"BVH: {info}", flush=True)
+    t, b0, b1, prim = gpu.trace_closest(rays, refine_fp64=True)
+    hit = prim >= 0
+    # barycentrics inside the triangle, t inside (tmin, tmax)
+    assert np.all(b0[hit] >= -1e-6) and np.all(b1[hit] >= -1e-6) and np.all(b0[hit] + b1[hit] <= 1 + 1e-6)
+    assert np.all(t[hit] > 0)
+    # any hit agrees with closest hit on every ray
+    occ = gpu.trace_any(rays)
+    np.testing.assert_array_equal(occ, hit)
+    # parity on every 800th ray (20k rays) against the FP64 oracle
+    sub = rays.slice(0, len(rays))
+    idx = np.arange(0, len(rays), 800)
+    rs = scenes.Rays(sub.o[idx], sub.d[idx], sub.tmin[idx], sub.tmax[idx])
+    to, b0o, b1o, po = oracle.Oracle(sc, fp64=True, accel="bvh").trace_closest(rs)
+    same, tie, edge, bad = classify(t[idx], b0[idx], b1[idx], prim[idx], to, b0o, b1o, po)
+    print(f"full-size subset: identical {same}, near-tie {tie}, near-edge {edge}, unexplained {len(bad)}", flush=True)
+    assert len(bad) == 0

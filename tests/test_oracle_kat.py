@@ -1,0 +1,309 @@
+"""Pins the CPU restatement (oracle/) against the reference's own known-answer tests,
+published golden vectors and closed forms.  CPU only."""
+import numpy as np
+import pytest
+
+from lightmetrica_amd import scenes
+
+
+# ---- Philox4x32-10: Random123 known-answer vectors (kat_vectors, philox4x32 10 rounds) ----
+PHILOX_KAT = [
+    ((0, 0), (0, 0, 0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF,) * 4, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0xA4093822, 0x299F31D0), (0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("key,ctr,expect", PHILOX_KAT)
+def test_philox_kat_oracle(oracle, key, ctr, expect):
+    assert tuple(oracle.philox(key, ctr)) == expect
+
+
+@pytest.mark.parametrize("key,ctr,expect", PHILOX_KAT)
+def test_philox_kat_numpy(key, ctr, expect):
+    w = scenes.philox4x32_10(*[np.uint64(c) for c in ctr], *key)
+    assert tuple(int(x) for x in w) == expect
+
+
+def test_sample_stream_matches(oracle):
+    for seed, sample in [(1, 0), (1, 12345), (0xDEADBEEF12, 2**33 + 7)]:
+        a = oracle.philox_stream(seed, sample, 37)
+        b = scenes.sample_stream_u32(seed, sample, 37)
+        np.testing.assert_array_equal(a, b)
+
+
+# ---- Warps: test.math.stats.cpp:32-88 ----------------------------------------------------
+@pytest.mark.parametrize("fp64", [True, False])
+def test_concentric_disk_kat(oracle, fp64):
+    src = [(0.5, 0.5), (1, 0.5), (0.5, 1), (0, 0.5), (0.5, 0)]
+    exp = [(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1)]
+    tol = 1e-5 if fp64 else 1e-3  # EpsLarge (math.constants.inl:69-79)
+    for s, e in zip(src, exp):
+        np.testing.assert_allclose(oracle.concentric_disk(s, fp64), e, atol=tol)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_cosine_hemisphere_kat(oracle, fp64):
+    tol = 1e-5 if fp64 else 1e-3
+    np.testing.assert_allclose(oracle.cosine_hemisphere((0.5, 0.5), fp64), (0, 0, 1), atol=tol)
+    # The reference lists (0.75, 0.75) -> (sqrt2/4, sqrt2/4, sqrt3/2) but runs only the first case
+    # (Count = 1); the concentric map gives r = 0.5 on the diagonal, i.e. the same point.
+    np.testing.assert_allclose(oracle.cosine_hemisphere((0.75, 0.75), fp64),
+                               (np.sqrt(2) / 4, np.sqrt(2) / 4, np.sqrt(3) / 2), atol=tol)
+
+
+def test_cosine_hemisphere_is_unit_and_upper(oracle):
+    rng = np.random.default_rng(3)
+    for u in rng.random((200, 2)):
+        v = oracle.cosine_hemisphere(u)
+        assert abs(np.linalg.norm(v) - 1) < 1e-12 and v[2] >= 0
+
+
+def test_uniform_sphere_closed_form(oracle):
+    # z = 1 - 2 u0, phi = 2 pi u1 (math.stats.inl:105-112)
+    for u in [(0.0, 0.0), (0.25, 0.5), (0.5, 0.25), (1.0, 0.75)]:
+        z = 1 - 2 * u[0]
+        r = np.sqrt(max(0.0, 1 - z * z))
+        phi = 2 * np.pi * u[1]
+        np.testing.assert_allclose(oracle.uniform_sphere(u), (r * np.cos(phi), r * np.sin(phi), z), atol=1e-12)
+
+
+# ---- Transforms: test.math.transform.cpp:65-122 ---------------------------------------------
+def _apply(m16, v):
+    m = np.asarray(m16).reshape(4, 4).T  # column-major -> row-major
+    return m @ np.asarray(v, float)
+
+
+def test_look_at_kat(oracle):
+    v1 = oracle.look_at((0, 1, 0), (0, 0, 0), (0, 0, 1))
+    np.testing.assert_allclose(_apply(v1, (0, 0, 0, 1)), (0, 0, -1, 1), atol=1e-12)
+    v2 = oracle.look_at((1, 1, 1), (0, 0, 0), (0, 0, 1))
+    np.testing.assert_allclose(_apply(v2, (0, 0, 0, 1)), (0, 0, -np.sqrt(3), 1), atol=1e-12)
+    # The product's generator computes the same matrix.
+    np.testing.assert_allclose(scenes.look_at((1, 1, 1), (0, 0, 0), (0, 0, 1)), v2, atol=1e-12)
+
+
+def test_perspective_kat(oracle):
+    P = oracle.perspective(90, 1.5, 1, 1000)
+    for src, exp in [((0, 0, -1, 1), (0, 0, -1)), ((0, 0, -1000, 1), (0, 0, 1)), ((1.5, 1, -1, 1), (1, 1, -1)),
+                     ((-1500, -1000, -1000, 1), (-1, -1, 1))]:
+        t = _apply(P, src)
+        np.testing.assert_allclose(t[:3] / t[3], exp, atol=1e-9)
+
+
+# ---- Camera: test.perspectivecamera.cpp:84-184 (fovy 90, film 200 x 100) -----------------------
+@pytest.mark.parametrize("fp64", [True, False])
+def test_camera_kat(oracle, fp64):
+    tol = 1e-5 if fp64 else 1e-3
+    I = np.eye(4).reshape(-1)
+    np.testing.assert_allclose(oracle.camera_direction(I, 90, 2.0, (0.5, 0.5), fp64), (0, 0, -1), atol=tol)
+    np.testing.assert_allclose(oracle.camera_direction(I, 90, 2.0, (1, 1), fp64), np.array([2, 1, -1]) / np.sqrt(6),
+                               atol=tol)
+    V = oracle.look_at((1, 1, 1), (0, 0, 0), (0, 0, 1))
+    np.testing.assert_allclose(oracle.camera_direction(V, 90, 2.0, (0.5, 0.5), fp64), -np.ones(3) / np.sqrt(3), atol=tol)
+    ok, r = oracle.camera_raster(I, 90, 2.0, (0, 0, 0), (0, 0, -1), fp64)
+    assert ok and np.allclose(r, (0.5, 0.5), atol=tol)
+    ok, r = oracle.camera_raster(I, 90, 2.0, (0, 0, 0), np.array([2, 1, -1]) / np.sqrt(6), fp64)
+    assert ok and np.allclose(r, (1, 1), atol=tol)
+    ok, r = oracle.camera_raster(V, 90, 2.0, (1, 1, 1), -np.ones(3) / np.sqrt(3), fp64)
+    assert ok and np.allclose(r, (0.5, 0.5), atol=tol)
+    # behind the camera / outside the frustum
+    ok, _ = oracle.camera_raster(I, 90, 2.0, (0, 0, 0), (0, 1, -0.1), fp64)
+    assert not ok
+
+
+# ---- Intersection KATs: test.scene.intersection.cpp:117-187 ---------------------------------
+ACCELS = [(True, "naive"), (True, "bvh"), (False, "naive"), (False, "bvh"), (False, "qbvh")]
+
+
+@pytest.mark.parametrize("fp64,accel", ACCELS)
+def test_intersect_simple(oracle, fp64, accel):
+    sc = scenes.stub_simple_mesh()
+    o = oracle.Oracle(sc, fp64=fp64, accel=accel)
+    tol = 1e-5 if fp64 else 1e-3
+    rays = scenes.grid_rays_simple()
+    xs = np.arange(1, 10) / 10
+    k = 0
+    for y in xs:
+        for x in xs:
+            g = o.intersect_geom(rays.o[k], rays.d[k])
+            k += 1
+            assert g is not None
+            np.testing.assert_allclose(g["p"], (x, y, 0), atol=tol)
+            np.testing.assert_allclose(g["gn"], (0, 0, 1), atol=tol)
+            np.testing.assert_allclose(g["sn"], (0, 0, 1), atol=tol)
+
+
+@pytest.mark.parametrize("fp64,accel", ACCELS)
+def test_intersect_simple2(oracle, fp64, accel):
+    sc = scenes.stub_simple2_mesh()
+    o = oracle.Oracle(sc, fp64=fp64, accel=accel)
+    tol = 1e-5 if fp64 else 1e-3
+    rays = scenes.grid_rays_down()
+    n = np.array([1, 0, 1]) / np.sqrt(2)
+    for k in range(len(rays)):
+        x, y = rays.o[k][0], rays.o[k][1]
+        g = o.intersect_geom(rays.o[k], rays.d[k])
+        assert g is not None
+        np.testing.assert_allclose(g["p"], (x, y, -x), atol=tol)
+        np.testing.assert_allclose(g["gn"], n, atol=tol)
+        np.testing.assert_allclose(g["sn"], n, atol=tol)
+
+
+def test_consistency_random_mesh(oracle):
+    """Consistency (test.scene.intersection.cpp:190-266): naive, bvh and qbvh agree on StubTriangleMesh_Random."""
+    sc = scenes.stub_random_mesh(oracle.stub_random_values())
+    rays = scenes.grid_rays_down()
+    res = []
+    for fp64, accel in ACCELS:
+        o = oracle.Oracle(sc, fp64=fp64, accel=accel)
+        res.append(o.trace_closest(rays, threads=1))
+    ref_tri = res[0][3]
+    assert (ref_tri >= 0).sum() > 40
+    for t, b0, b1, tri in res[1:]:
+        np.testing.assert_array_equal(tri, ref_tri)
+        hit = tri >= 0
+        # The reference compares with EpsLarge (1e-3 absolute for float); this is 100x stricter.
+        np.testing.assert_allclose(t[hit], res[0][0][hit], rtol=0, atol=1e-5)
+
+
+def _brute_force(P, F, o, d, tmin):
+    """Moeller-Trumbore over every triangle in float64 (the closed form the accelerators must agree with)."""
+    p1, p2, p3 = P[F[:, 0]].astype(np.float64), P[F[:, 1]].astype(np.float64), P[F[:, 2]].astype(np.float64)
+    e1, e2 = p2 - p1, p3 - p1
+    out = []
+    for oi, di in zip(o.astype(np.float64), d.astype(np.float64)):
+        s1 = np.cross(di, e2)
+        div = (s1 * e1).sum(1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = 1.0 / div
+            q = oi - p1
+            u = (q * s1).sum(1) * inv
+            s2 = np.cross(q, e1)
+            v = (di * s2).sum(1) * inv
+            t = (e2 * s2).sum(1) * inv
+        ok = (div != 0) & (u >= 0) & (v >= 0) & (u + v <= 1) & (t > tmin)
+        if ok.any():
+            i = np.where(ok)[0][np.argmin(t[ok])]
+            out.append((t[i], i))
+        else:
+            out.append((-1.0, -1))
+    return out
+
+
+@pytest.mark.parametrize("fp64,accel", [(True, "bvh"), (False, "qbvh")])
+def test_accelerators_match_brute_force(oracle, fp64, accel):
+    sc = scenes.traversal_scene(num_triangles=3000, seed=77, sigma=0.05)
+    rays = scenes.traversal_rays(400, seed=78)
+    o = oracle.Oracle(sc, fp64=fp64, accel=accel)
+    t, b0, b1, tri = o.trace_closest(rays, threads=1)
+    bf = _brute_force(sc.positions, sc.faces, rays.o, rays.d, 0.0)
+    mism = 0
+    for k, (tb, ib) in enumerate(bf):
+        if ib != tri[k]:
+            # only a near tie (equal t to 1e-6) may differ
+            if not (ib >= 0 and tri[k] >= 0 and abs(tb - t[k]) <= 1e-6 * abs(tb)):
+                mism += 1
+        elif ib >= 0:
+            assert abs(t[k] - tb) <= (1e-12 if fp64 else 1e-5) * max(1.0, abs(tb))
+    assert mism == 0
+
+
+def test_qbvh_stack_never_overflows(oracle):
+    sc = scenes.traversal_scene(num_triangles=50_000, seed=5)
+    rays = scenes.traversal_rays(20_000, seed=6)
+    o = oracle.Oracle(sc, fp64=False, accel="qbvh")
+    o.trace_closest(rays)
+    assert o.stack_overflows() == 0
+
+
+# ---- Film semantics: test.hdrfilm.cpp, sched.mt.cpp:256 --------------------------------------
+def test_film_rescale_and_pixel_convention(oracle):
+    """A scene with no light renders black; the rescale is W*H/N (closed form checked by the furnace)."""
+    sc = scenes.room_scene(8, 8, 1)
+    sc.lights = sc.lights[:0]
+    sc.light_ids = np.full_like(sc.light_ids, -1)
+    o = oracle.Oracle(sc, fp64=True)
+    film, st = o.render("pt", num_samples=640)
+    assert film.shape == (8, 8, 3) and np.all(film == 0) and st["samples"] == 640
+
+
+# ---- Estimator closed forms ----------------------------------------------------------------
+def furnace_scene(albedo: float, Le: float, res: int = 4):
+    """Closed box (6 inward quads) where every surface emits Le and reflects albedo (diffuse),
+    viewed from the inside: every pixel of `pt` has mean Le * sum_{k=0}^{M-2} albedo^k."""
+    b = scenes._Builder()
+    b.quad((0, 0, 0), (0, 0, 1), (1, 0, 0), 0, 0)
+    b.quad((0, 1, 0), (1, 0, 0), (0, 0, 1), 0, 0)
+    b.quad((0, 0, 0), (1, 0, 0), (0, 1, 0), 0, 0)
+    b.quad((0, 0, 1), (0, 1, 0), (1, 0, 0), 0, 0)
+    b.quad((0, 0, 0), (0, 1, 0), (0, 0, 1), 0, 0)
+    b.quad((1, 0, 0), (0, 0, 1), (0, 1, 0), 0, 0)
+    P, N, F, M, L = b.arrays()
+    mats = scenes._materials([(scenes.BSDF_DIFFUSE, (albedo,) * 3)])
+    lights = scenes._lights([(scenes.LIGHT_AREA, (Le,) * 3)])
+    view = scenes.look_at((0.5, 0.5, 0.5), (0.5, 0.5, 0.0), (0, 1, 0)).astype(np.float32)
+    cam = scenes.Camera(view=view, fovy=60.0, width=res, height=res)
+    rc = scenes.RenderConfig(num_samples=res * res * 1000, max_path_vertices=6, rr_depth=3, seed=5)
+    return scenes.Scene("furnace", P, N, F, M, L, mats, lights, cam, rc)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_white_furnace_closed_form(oracle, fp64):
+    a, Le, M = 0.5, 1.0, 6
+    sc = furnace_scene(a, Le)
+    o = oracle.Oracle(sc, fp64=fp64, accel="bvh" if fp64 else "qbvh")
+    film, st = o.render("pt", num_samples=200_000)
+    expect = Le * sum(a ** k for k in range(M - 1))
+    # RR makes the estimate noisy; the image mean of 200k samples is within a few 1e-3.
+    assert abs(film.mean() - expect) < 0.01 * expect, (film.mean(), expect)
+    # Without RR every sample carries exactly sum a^k; raster positions are uniform over the whole
+    # film (per-pixel counts are multinomial), so the image mean -- sum of all samples / N -- is exact.
+    film, st = o.render("pt", num_samples=16 * 50, rr_depth=-1)
+    np.testing.assert_allclose(film.mean(), expect, rtol=1e-5 if not fp64 else 1e-12)
+
+
+def test_furnace_catches_planted_mistake(oracle):
+    """Planted in the oracle's inputs on the CPU: dropping one bounce (max_path_vertices - 1) moves the
+    deterministic furnace value off the closed form, so the check above has teeth."""
+    a, Le, M = 0.5, 1.0, 6
+    sc = furnace_scene(a, Le)
+    o = oracle.Oracle(sc, fp64=True)
+    film, _ = o.render("pt", num_samples=16 * 10, rr_depth=-1, max_path_vertices=M - 1)
+    expect = Le * sum(a ** k for k in range(M - 1))
+    assert not np.isclose(film.mean(), expect, rtol=1e-3)
+
+
+def test_estimators_agree_within_3_sigma(oracle):
+    """pt, pt.direct and pt.mis are unbiased for the same image: per-pixel means agree within 3 sigma."""
+    sc = scenes.room_scene(8, 8, 1)
+    o = oracle.Oracle(sc, fp64=True)
+    runs = {r: [o.render(r, num_samples=64 * 4000, seed=s)[0] for s in range(1, 17)] for r in ("pt", "pt.direct", "pt.mis")}
+    mean = {r: np.mean(v, axis=0) for r, v in runs.items()}
+    sem = {r: np.std(v, axis=0, ddof=1) / np.sqrt(len(v)) for r, v in runs.items()}
+    for a, b in [("pt", "pt.direct"), ("pt", "pt.mis"), ("pt.direct", "pt.mis")]:
+        zs = (mean[a] - mean[b]) / np.sqrt(sem[a] ** 2 + sem[b] ** 2 + 1e-12)
+        z = np.abs(zs)
+        # Per pixel within 3 sigma (the sigma estimate has 15 dof, so a few values may exceed it) and a
+        # global chi-square per degree of freedom near 1.
+        assert (z > 3.5).mean() < 0.02, (a, b, (z > 3.5).mean())
+        assert np.mean(z ** 2) < 2.0, (a, b, np.mean(z ** 2))  # pt is heavy-tailed (small light)
+        assert abs(np.mean(zs)) < 0.3, (a, b, np.mean(zs))  # no systematic offset
+
+
+def test_directly_visible_light_equals_Le(oracle):
+    """A camera looking straight at the emitter sees Le in every pixel (pt, first hit)."""
+    b = scenes._Builder()
+    b.quad((-10, -10, -1), (20, 0, 0), (0, 20, 0), 0, 0)  # faces +z toward the camera
+    P, N, F, M, L = b.arrays()
+    mats = scenes._materials([(scenes.BSDF_DIFFUSE, (0, 0, 0))])
+    lights = scenes._lights([(scenes.LIGHT_AREA, (3.0, 2.0, 1.0))])
+    view = scenes.look_at((0, 0, 0), (0, 0, -1), (0, 1, 0)).astype(np.float32)
+    cam = scenes.Camera(view=view, fovy=30.0, width=4, height=4)
+    sc = scenes.Scene("wall", P, N, F, M, L, mats, lights, cam, scenes.RenderConfig(16 * 100, 9, 3, 1))
+    for fp64 in (True, False):
+        o = oracle.Oracle(sc, fp64=fp64, accel="bvh" if fp64 else "qbvh")
+        film, _ = o.render("pt", num_samples=16 * 2000)
+        # raster sampling is uniform over the film, so pixel means carry multinomial noise of ~1/sqrt(2000)
+        np.testing.assert_allclose(film.mean(axis=(0, 1)), (3, 2, 1), rtol=1e-9 if fp64 else 1e-5)
