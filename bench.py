@@ -198,7 +198,20 @@ def main():
     ap.add_argument("--no-pt", action="store_true")
     ap.add_argument("--cpu-sample-rays", type=int, default=4_000_000)
     ap.add_argument("--cpu-sample-paths", type=int, default=1_048_576)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the path shards by sample index; see DESIGN.md).", file=sys.stderr)

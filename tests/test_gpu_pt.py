@@ -110,7 +110,10 @@ def test_furnace_closed_form(gpu):
     gpu.build_bvh()
     expect = Le * sum(a ** k for k in range(M - 1))
     film, _ = gpu.render("pt", num_samples=256 * 100, rr_depth=-1, film_precision="fp64")
-    np.testing.assert_allclose(film.mean(), expect, rtol=1e-5)
+    # A bounce ray that starts within Eps (1e-4) of the adjacent wall skips it and leaves through
+    # the corner (the reference's minT = Eps semantics, pathtrace.cpp:245): a few paths in 1e4.
+    np.testing.assert_allclose(film.mean(), expect, rtol=2e-4)
+    assert film.mean() <= expect + 1e-9
     film, _ = gpu.render("pt", num_samples=256 * 2000, film_precision="fp64")
     assert abs(film.mean() - expect) < 0.01 * expect
 
