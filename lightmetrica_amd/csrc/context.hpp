@@ -117,6 +117,7 @@ struct lmgpu_ctx {
     void* trace_staging = nullptr;
     size_t trace_staging_bytes = 0;
     uint64_t* count_buf = nullptr;  // 2 x u64 for lmgpu_trace_count
+    uint32_t* next_counter = nullptr;  // work counter of the persistent traversal kernels
 
     // Path-tracing work buffers (capacity = wavefront width).
     uint32_t pt_capacity = 0;

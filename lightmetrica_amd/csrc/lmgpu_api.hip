@@ -157,6 +157,13 @@ lmgpu_status lmgpu_create(int device, lmgpu_ctx** out) {
     }
     ctx->stream = ctx->own_stream;
     ctx->env.index = -1;
+    void* counters = nullptr;
+    if (hipMalloc(&counters, 256) != hipSuccess) {
+        lmgpu_destroy(ctx);
+        return LMGPU_ERR_OUT_OF_MEMORY;
+    }
+    ctx->count_buf = static_cast<uint64_t*>(counters);
+    ctx->next_counter = reinterpret_cast<uint32_t*>(static_cast<char*>(counters) + 128);
     *out = ctx;
     return LMGPU_OK;
 }
@@ -516,12 +523,6 @@ lmgpu_status lmgpu_trace_count(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint3
     lmgpu_status s = check_trace(ctx, rays, node_visits);
     if (s != LMGPU_OK) return s;
     if (!tri_tests) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_trace_count: NULL output");
-    if (!ctx->count_buf) {
-        void* p = nullptr;
-        s = dev_alloc(ctx, &p, 16, "counters");
-        if (s != LMGPU_OK) return s;
-        ctx->count_buf = static_cast<uint64_t*>(p);
-    }
     const float* dev[8];
     char* out = nullptr;
     s = prepare_rays(ctx, rays, n, 0, dev, &out);

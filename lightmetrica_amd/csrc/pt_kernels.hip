@@ -71,7 +71,8 @@ struct PTBuffers {
     Queue q[2];
     float4* hits;  // t, b0, b1, prim (bits)
     ShadowQueue sq;
-    uint32_t* counters;  // [0, 64): live-queue sizes per segment, [64, 128): shadow sizes per segment
+    uint32_t* counters;  // [0, 64): live-queue sizes per segment, [64, 128): shadow sizes per segment,
+                         // [128, 192): work counters of the extension launches, [192, 257): of the shadow launches
     unsigned long long* totals;  // closest rays, shadow rays
 };
 
@@ -89,6 +90,45 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     if (cnt && int(lane) == leader) base = atomicAdd(counter, cnt);
     base = __shfl(base, leader, 64);
     return base + before;
+}
+
+// Block-aggregated queue append: one atomic per block of kPTBlock threads instead of one
+// per wavefront (same-address atomics serialise in L2).  Every thread of the block must call
+// it -- it synchronises the block -- and each call site of a kernel needs its own kSite.
+template <int kSite>
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred) {
+    __shared__ uint32_t wave_off[kPTBlock / 64];
+    __shared__ uint32_t base;
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (lane == 0) wave_off[wave] = uint32_t(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kPTBlock / 64; w++) {
+            const uint32_t c = wave_off[w];
+            wave_off[w] = t;
+            t += c;
+        }
+        base = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    return base + wave_off[wave] + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ void film_add_raw(void* film, int fp64, uint32_t pixel, V3 v) {
+    if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return;
+    if (fp64) {
+        double* f = static_cast<double*>(film) + 3ull * pixel;
+        unsafeAtomicAdd(f + 0, double(v.x));
+        unsafeAtomicAdd(f + 1, double(v.y));
+        unsafeAtomicAdd(f + 2, double(v.z));
+    } else {
+        float* f = static_cast<float*>(film) + 3ull * pixel;
+        unsafeAtomicAdd(f + 0, v.x);
+        unsafeAtomicAdd(f + 1, v.y);
+        unsafeAtomicAdd(f + 2, v.z);
+    }
 }
 
 __device__ __forceinline__ void film_add(const PTConsts& c, uint32_t pixel, V3 v) {
@@ -274,9 +314,15 @@ struct NEEVertex {
     V3 wi;
 };
 
-__device__ __forceinline__ void nee(const PTConsts& c, const NEEVertex& v, V3 T, int n, uint32_t pixel,
-                                    SampleStream& rng, bool mis, ShadowQueue sq, uint32_t* shadow_count,
-                                    bool active) {
+struct ShadowRec {
+    bool emit;
+    float4 o, d, c;
+};
+
+// Returns the shadow-ray record of the connection (emit = false when the contribution is 0;
+// the caller appends emitted records to the shadow queue).
+__device__ __forceinline__ ShadowRec nee(const PTConsts& c, const NEEVertex& v, V3 T, int n, uint32_t pixel,
+                                         SampleStream& rng, bool mis, bool active) {
     V3 contrib = v3s(0.f);
     V3 target = v.p;
     uint32_t pix = pixel;
@@ -359,17 +405,24 @@ __device__ __forceinline__ void nee(const PTConsts& c, const NEEVertex& v, V3 T,
             target = lp;
         }
     }
-    const bool emit = active && !is_zero(contrib);
-    const uint32_t slot = wave_append(shadow_count, emit);
-    if (emit) {
+    ShadowRec rec;
+    rec.emit = active && !is_zero(contrib);
+    if (rec.emit) {
         // RenderUtils::Visible: d = (p2 - p1) / |p2 - p1|, (Eps, |p2 - p1| (1 - Eps))
         const V3 dv = target - v.p;
         const float len = length(dv);
         const V3 d = dv / len;
-        sq.o[slot] = make_float4(v.p.x, v.p.y, v.p.z, len * (1.f - kEps));
-        sq.d[slot] = make_float4(d.x, d.y, d.z, __uint_as_float(pix));
-        sq.c[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
+        rec.o = make_float4(v.p.x, v.p.y, v.p.z, len * (1.f - kEps));
+        rec.d = make_float4(d.x, d.y, d.z, __uint_as_float(pix));
+        rec.c = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
     }
+    return rec;
+}
+
+__device__ __forceinline__ void put_shadow(const ShadowQueue& sq, uint32_t slot, const ShadowRec& r) {
+    sq.o[slot] = r.o;
+    sq.d[slot] = r.d;
+    sq.c[slot] = r.c;
 }
 
 // ---------------------------------------------------------------------------------
@@ -389,32 +442,35 @@ __device__ __forceinline__ void store_path(const Queue& q, uint32_t slot, V3 o, 
 __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_t* live_count, ShadowQueue sq,
                                                   uint32_t* shadow_count) {
     const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
-    if (i >= c.batch) return;
+    const bool valid = i < c.batch;
     SampleStream rng;
     rng.init(c.seed, c.base + i, 0);
     const V3 cam_p = v3(c.cam.pos[0], c.cam.pos[1], c.cam.pos[2]);
+    const bool start = !(c.max_pv != -1 && 1 >= c.max_pv);
     if (c.renderer == LMGPU_PT) {
+        // Every path of pt survives its camera vertex: slot = sample index, no compaction.
+        if (!valid || !start) return;
+        if (i == 0) *live_count = c.batch;
         const float r0 = rng.next(), r1 = rng.next();
         rng.next();  // Camera::SamplePosition draws a Vec2 (pathtrace.cpp:173)
         rng.next();
         float pdf;
         const V3 d = cam_direction(c.cam, r0, r1, pdf);
-        const bool alive = !(c.max_pv != -1 && 1 >= c.max_pv);
-        const uint32_t slot = wave_append(live_count, alive);
-        if (alive)
-            store_path(out, slot, cam_p, 0.f, d, pdf, v3s(1.f), v3s(0.f), i, 1u, rng.dim, pixel_of(c.cam, r0, r1));
+        store_path(out, i, cam_p, 0.f, d, pdf, v3s(1.f), v3s(0.f), i, 1u, rng.dim, pixel_of(c.cam, r0, r1));
         return;
     }
     const bool mis = c.renderer == LMGPU_PT_MIS;
     rng.next();  // Camera::SamplePosition (pathtrace.direct.cpp:175)
     rng.next();
     V3 T = v3s(1.f);
-    bool alive = !(c.max_pv != -1 && 1 >= c.max_pv);
+    bool alive = valid && start;
     NEEVertex v;
     v.p = cam_p;
     v.camera = true;
     v.wi = v3s(0.f);
-    nee(c, v, T, 1, 0u, rng, mis, sq, shadow_count, alive);
+    const ShadowRec rec = nee(c, v, T, 1, 0u, rng, mis, alive);
+    const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
+    if (rec.emit) put_shadow(sq, sslot, rec);
     if (alive && c.rr_depth != -1 && 1 >= c.rr_depth) {
         const float q = fminf(0.5f, luminance(T));
         if (rng.next() > q) alive = false;
@@ -431,33 +487,43 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
         if (!cam_ray_to_raster(c.cam, cam_p, d, rx, ry)) alive = false;
         pixel = pixel_of(c.cam, rx, ry);
     }
-    const uint32_t slot = wave_append(live_count, alive);
+    const uint32_t slot = block_append<1>(live_count, alive);
     if (alive)
         store_path(out, slot, cam_p, kEps, d, pdf, T, v3s(0.f), i, 1u | (kFlagDegenerate << 16), rng.dim, pixel);
 }
 
-// Extension rays: closest hit of every live path.
+// Extension rays: closest hit of every live path (persistent, refilling lanes).
+struct QueueSource {
+    Queue q;
+    __device__ __forceinline__ RayIn operator()(uint32_t i) const {
+        const float4 o = q.ray_o[i], d = q.ray_d[i];
+        return RayIn{o.x, o.y, o.z, d.x, d.y, d.z, o.w, kInf};
+    }
+};
+
+struct HitSink {
+    const TriRecord* tris;
+    float4* hits;
+    __device__ __forceinline__ void operator()(uint32_t i, bool found, const HitOut& h, const RayIn& r) const {
+        float4 out = make_float4(0.f, 0.f, 0.f, __uint_as_float(kMiss));
+        if (found) {
+            float t, b0, b1;
+            winner_barycentrics(tris, r, h, false, t, b0, b1);
+            out = make_float4(t, b0, b1, __uint_as_float(h.prim));
+        }
+        hits[i] = out;
+    }
+};
+
 __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict__ nodes,
                                                         const TriRecord* __restrict__ tris, Queue q,
-                                                        const uint32_t* __restrict__ count, float4* __restrict__ hits) {
+                                                        const uint32_t* __restrict__ count, uint32_t* next,
+                                                        float4* __restrict__ hits) {
     __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= *count) return;
-    const float4 o = q.ray_o[i], d = q.ray_d[i];
-    RayIn r{o.x, o.y, o.z, d.x, d.y, d.z, o.w, kInf};
-    Stack st;
-    st.lds = lds_stack;
-    st.lane = int(threadIdx.x);
-    st.overflow = false;
-    HitOut h;
-    TraceCounters cnt{0, 0};
-    float4 out = make_float4(0.f, 0.f, 0.f, __uint_as_float(kMiss));
-    if (traverse<false, false>(nodes, tris, r, st, h, cnt)) {
-        float t, b0, b1;
-        winner_barycentrics(tris, r, h, false, t, b0, b1);
-        out = make_float4(t, b0, b1, __uint_as_float(h.prim));
-    }
-    hits[i] = out;
+    const uint32_t n = *count;
+    QueueSource src{q};
+    HitSink sink{tris, hits};
+    trace_persistent<false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink);
 }
 
 // pt: pathtrace.cpp:203-262 for one segment.
@@ -465,9 +531,10 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
                                                        const float4* __restrict__ hits, Queue out,
                                                        uint32_t* out_count) {
     const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
-    if (i >= *in_count) return;
-    const float4 ro = in.ray_o[i], rd = in.ray_d[i], th = in.thr[i], ax = in.aux[i];
-    const uint4 meta = in.meta[i];
+    const bool valid = i < *in_count;
+    const uint32_t j = valid ? i : 0u;  // invalid lanes read slot 0 and discard it
+    const float4 ro = in.ray_o[j], rd = in.ray_d[j], th = in.thr[j], ax = in.aux[j];
+    const uint4 meta = in.meta[j];
     const V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
     V3 T = v3(th.x, th.y, th.z), L = v3(ax.x, ax.y, ax.z);
     int n = int(meta.y & 0xffffu);
@@ -477,7 +544,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
     float pdf = 0.f;
     SampleStream rng;
     rng.init(c.seed, c.base + meta.x, meta.z);
-    if (surface_at(c, o, d, ro.w, hits[i], s)) {
+    if (valid && surface_at(c, o, d, ro.w, hits[j], s)) {
         if (s.light >= 0 && dot(s.f.sn, -d) > 0.f) L = L + T * v3s(kInvPi) * (light_Le(c, s.light) * kPi);
         const float u0 = rng.next(), u1 = rng.next(), uc = rng.next();
         int type;
@@ -494,9 +561,9 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
             if (c.max_pv != -1 && n >= c.max_pv) alive = false;
         }
     }
-    const uint32_t slot = wave_append(out_count, alive);
+    const uint32_t slot = block_append<0>(out_count, alive);
     if (alive) store_path(out, slot, s.p, kEps, wo, pdf, T, L, meta.x, uint32_t(n), rng.dim, meta.w);
-    else film_add(c, meta.w, L);
+    else if (valid) film_add(c, meta.w, L);
 }
 
 // pt.direct / pt.mis: the hit half of one loop iteration (pathtrace.direct.cpp:304-336,
@@ -506,10 +573,11 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
                                                         const float4* __restrict__ hits, Queue out,
                                                         uint32_t* out_count, ShadowQueue sq, uint32_t* shadow_count) {
     const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
-    if (i >= *in_count) return;
+    const bool valid = i < *in_count;
+    const uint32_t j = valid ? i : 0u;  // invalid lanes read slot 0 and discard it
     const bool mis = c.renderer == LMGPU_PT_MIS;
-    const float4 ro = in.ray_o[i], rd = in.ray_d[i], th = in.thr[i], ax = in.aux[i];
-    const uint4 meta = in.meta[i];
+    const float4 ro = in.ray_o[j], rd = in.ray_d[j], th = in.thr[j], ax = in.aux[j];
+    const uint4 meta = in.meta[j];
     const V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
     V3 T = v3(th.x, th.y, th.z);
     int n = int(meta.y & 0xffffu);
@@ -517,7 +585,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     SampleStream rng;
     rng.init(c.seed, c.base + meta.x, meta.z);
     Surf s;
-    bool alive = surface_at(c, o, d, ro.w, hits[i], s);
+    bool alive = valid && surface_at(c, o, d, ro.w, hits[j], s);
     if (alive && s.light >= 0) {
         const float LeD = dot(s.f.sn, -d) > 0.f ? kInvPi : 0.f;
         const V3 LeP = light_Le(c, s.light) * kPi;
@@ -550,7 +618,9 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     v.camera = false;
     v.wi = -d;
     v.m = alive ? mat_info(c, s.mat) : MatInfo{LMGPU_BSDF_MIRROR, v3s(0.f)};
-    nee(c, v, T, n, meta.w, rng, mis, sq, shadow_count, alive && mat_nondelta(v.m));
+    const ShadowRec rec = nee(c, v, T, n, meta.w, rng, mis, alive && mat_nondelta(v.m));
+    const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
+    if (rec.emit) put_shadow(sq, sslot, rec);
     if (alive && c.rr_depth != -1 && n >= c.rr_depth) {
         const float q = fminf(0.5f, luminance(T));
         if (rng.next() > q) alive = false;
@@ -565,7 +635,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
         if (is_zero(fs)) alive = false;
         else T = T * fs;
     }
-    const uint32_t slot = wave_append(out_count, alive);
+    const uint32_t slot = block_append<1>(out_count, alive);
     if (alive) {
         const uint32_t nf = uint32_t(n) | ((type & (1 << 2)) ? (kFlagSpecular << 16) : 0u);
         store_path(out, slot, s.p, kEps, wo, pdf, T, s.f.sn, meta.x, nf, rng.dim, meta.w);
@@ -573,22 +643,32 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
 }
 
 // NEE shadow rays: any hit over the triangles, then the film update when unoccluded.
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue sq, const uint32_t* __restrict__ count) {
-    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= *count) return;
-    const float4 o = sq.o[i], d = sq.d[i];
-    RayIn r{o.x, o.y, o.z, d.x, d.y, d.z, kEps, o.w};
-    Stack st;
-    st.lds = lds_stack;
-    st.lane = int(threadIdx.x);
-    st.overflow = false;
-    HitOut h;
-    TraceCounters cnt{0, 0};
-    if (!traverse<true, false>(c.nodes, c.tris, r, st, h, cnt)) {
-        const float4 v = sq.c[i];
-        film_add(c, __float_as_uint(d.w), v3(v.x, v.y, v.z));
+struct ShadowSource {
+    ShadowQueue sq;
+    __device__ __forceinline__ RayIn operator()(uint32_t i) const {
+        const float4 o = sq.o[i], d = sq.d[i];
+        return RayIn{o.x, o.y, o.z, d.x, d.y, d.z, kEps, o.w};
     }
+};
+
+struct ShadowSink {
+    ShadowQueue sq;
+    void* film;
+    int fp64;
+    __device__ __forceinline__ void operator()(uint32_t i, bool found, const HitOut&, const RayIn&) const {
+        if (found) return;
+        const float4 v = sq.c[i];
+        film_add_raw(film, fp64, __float_as_uint(sq.d[i].w), v3(v.x, v.y, v.z));
+    }
+};
+
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue sq, const uint32_t* __restrict__ count,
+                                                        uint32_t* next) {
+    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    const uint32_t n = *count;
+    ShadowSource src{sq};
+    ShadowSink sink{sq, c.film, c.film_fp64};
+    trace_persistent<true>(c.nodes, c.tris, n, next, lds_stack + threadIdx.x, src, sink);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -727,6 +807,10 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
 
     hipStream_t s = ctx->stream;
     hipError_t e;
+    int occ_e = 0, occ_s = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_e, k_extend, kTraceBlock, 0) != hipSuccess || occ_e < 1) occ_e = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_shadow, kTraceBlock, 0) != hipSuccess || occ_s < 1) occ_s = 1;
+    const uint32_t trace_grid = uint32_t(std::max(ctx->num_cus, 1)) * uint32_t(std::min(occ_e, occ_s));
     hipEvent_t r0, r1;
     (void)hipEventCreate(&r0);
     (void)hipEventCreate(&r1);
@@ -741,8 +825,10 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
         const uint32_t nb = uint32_t(std::min<uint64_t>(cap, p->num_samples - base));
         c.base = p->sample_offset + base;
         c.batch = nb;
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, b.counters, 2u * kMaxSegments);
+        hipLaunchKernelGGL(k_zero_u32, dim3(2), dim3(256), 0, s, b.counters, 512u);
         const dim3 grid_b((nb + kPTBlock - 1) / kPTBlock), grid_c((cap + kPTBlock - 1) / kPTBlock);
+        // Persistent traversal grids: the device fill, never more than the queue capacity needs.
+        const dim3 grid_t(std::min<uint32_t>(grid_c.x, trace_grid));
         // counters[k]: queue entering segment k; counters[kMaxSegments + k]: shadow rays emitted before segment k.
         e = timed(ctx, kKGen, [&]() {
             hipLaunchKernelGGL(k_gen, grid_b, dim3(kPTBlock), 0, s, c, b.q[0], b.counters + 0, b.sq,
@@ -751,7 +837,8 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
         if (e != hipSuccess) return hip_fail(ctx, e, "k_gen");
         if (!is_pt) {
             e = timed(ctx, kKShadow, [&]() {
-                hipLaunchKernelGGL(k_shadow, grid_c, dim3(kTraceBlock), 0, s, c, b.sq, b.counters + kMaxSegments + 0);
+                hipLaunchKernelGGL(k_shadow, grid_t, dim3(kTraceBlock), 0, s, c, b.sq, b.counters + kMaxSegments + 0,
+                                   b.counters + 3 * kMaxSegments + 0);
             });
             if (e != hipSuccess) return hip_fail(ctx, e, "k_shadow");
         }
@@ -759,8 +846,8 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
             const Queue& qi = b.q[k & 1];
             const Queue& qo = b.q[(k + 1) & 1];
             e = timed(ctx, kKExtend, [&]() {
-                hipLaunchKernelGGL(k_extend, grid_c, dim3(kTraceBlock), 0, s, c.nodes, c.tris, qi, b.counters + k,
-                                   b.hits);
+                hipLaunchKernelGGL(k_extend, grid_t, dim3(kTraceBlock), 0, s, c.nodes, c.tris, qi, b.counters + k,
+                                   b.counters + 2 * kMaxSegments + k, b.hits);
             });
             if (e != hipSuccess) return hip_fail(ctx, e, "k_extend");
             e = timed(ctx, kKShade, [&]() {
@@ -774,8 +861,8 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
             if (e != hipSuccess) return hip_fail(ctx, e, "k_shade");
             if (!is_pt) {
                 e = timed(ctx, kKShadow, [&]() {
-                    hipLaunchKernelGGL(k_shadow, grid_c, dim3(kTraceBlock), 0, s, c, b.sq,
-                                       b.counters + kMaxSegments + k + 1);
+                    hipLaunchKernelGGL(k_shadow, grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
+                                       b.counters + kMaxSegments + k + 1, b.counters + 3 * kMaxSegments + k + 1);
                 });
                 if (e != hipSuccess) return hip_fail(ctx, e, "k_shadow");
             }

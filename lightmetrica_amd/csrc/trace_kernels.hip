@@ -2,6 +2,8 @@
 #include "context.hpp"
 #include "traverse.hpp"
 
+#include <algorithm>
+
 namespace lmgpu {
 
 struct RaysSoA {
@@ -16,51 +18,62 @@ __device__ __forceinline__ RayIn load_ray(const RaysSoA& r, uint32_t i) {
     return x;
 }
 
+struct SoASource {
+    RaysSoA rays;
+    __device__ __forceinline__ RayIn operator()(uint32_t i) const { return load_ray(rays, i); }
+};
+
+template <bool kRefine> struct ClosestSink {
+    const TriRecord* tris;
+    float *t_out, *b0_out, *b1_out;
+    int32_t* prim_out;
+    __device__ __forceinline__ void operator()(uint32_t i, bool found, const HitOut& h, const RayIn& r) const {
+        if (found) {
+            float t, b0, b1;
+            winner_barycentrics(tris, r, h, kRefine, t, b0, b1);
+            t_out[i] = t;
+            b0_out[i] = b0;
+            b1_out[i] = b1;
+            prim_out[i] = int32_t(h.prim);
+        } else {
+            t_out[i] = -1.f;
+            b0_out[i] = 0.f;
+            b1_out[i] = 0.f;
+            prim_out[i] = -1;
+        }
+    }
+};
+
+struct OcclusionSink {
+    uint8_t* occluded;
+    __device__ __forceinline__ void operator()(uint32_t i, bool found, const HitOut&, const RayIn&) const {
+        occluded[i] = found ? 1 : 0;
+    }
+};
+
 template <bool kRefine>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(const Node4* __restrict__ nodes,
                                                                const TriRecord* __restrict__ tris, RaysSoA rays,
-                                                               uint32_t n, float* __restrict__ t_out,
+                                                               uint32_t n, uint32_t* next, float* __restrict__ t_out,
                                                                float* __restrict__ b0_out, float* __restrict__ b1_out,
                                                                int32_t* __restrict__ prim_out) {
     __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= n) return;
-    const RayIn r = load_ray(rays, i);
-    Stack st;
-    st.lds = lds_stack;
-    st.lane = int(threadIdx.x);
-    st.overflow = false;
-    HitOut h;
-    TraceCounters cnt{0, 0};
-    if (traverse<false, false>(nodes, tris, r, st, h, cnt)) {
-        float t, b0, b1;
-        winner_barycentrics(tris, r, h, kRefine, t, b0, b1);
-        t_out[i] = t;
-        b0_out[i] = b0;
-        b1_out[i] = b1;
-        prim_out[i] = int32_t(h.prim);
-    } else {
-        t_out[i] = -1.f;
-        b0_out[i] = 0.f;
-        b1_out[i] = 0.f;
-        prim_out[i] = -1;
-    }
+    SoASource src{rays};
+    ClosestSink<kRefine> sink{tris, t_out, b0_out, b1_out, prim_out};
+    trace_persistent<false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_any(const Node4* __restrict__ nodes,
                                                            const TriRecord* __restrict__ tris, RaysSoA rays,
-                                                           uint32_t n, uint8_t* __restrict__ occluded) {
+                                                           uint32_t n, uint32_t* next, uint8_t* __restrict__ occluded) {
     __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= n) return;
-    const RayIn r = load_ray(rays, i);
-    Stack st;
-    st.lds = lds_stack;
-    st.lane = int(threadIdx.x);
-    st.overflow = false;
-    HitOut h;
-    TraceCounters cnt{0, 0};
-    occluded[i] = traverse<true, false>(nodes, tris, r, st, h, cnt) ? 1 : 0;
+    SoASource src{rays};
+    OcclusionSink sink{occluded};
+    trace_persistent<true>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink);
+}
+
+__global__ void k_zero_next(uint32_t* p) {
+    if (threadIdx.x == 0) *p = 0u;
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_count(const Node4* __restrict__ nodes,
@@ -71,12 +84,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_count(const Node4* __rest
     TraceCounters cnt{0, 0};
     if (i < n) {
         const RayIn r = load_ray(rays, i);
-        Stack st;
-        st.lds = lds_stack;
-        st.lane = int(threadIdx.x);
-        st.overflow = false;
         HitOut h;
-        traverse<false, true>(nodes, tris, r, st, h, cnt);
+        traverse<false, true>(nodes, tris, r, lds_stack + threadIdx.x, h, cnt);
     }
     // Wave-level reduction, one atomic per wave per counter.
     unsigned long long a = cnt.nodes, b = cnt.tris;
@@ -92,24 +101,37 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_count(const Node4* __rest
 
 static RaysSoA soa_of(const float* const* r) { return RaysSoA{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]}; }
 
+// Persistent grid: every CU filled to its occupancy limit, never more blocks than the batch needs.
+template <class K> static uint32_t persistent_blocks(lmgpu_ctx* ctx, K kernel, uint32_t n) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const uint64_t full = uint64_t(std::max(ctx->num_cus, 1)) * uint64_t(per_cu);
+    const uint64_t need = (uint64_t(n) + kTraceBlock - 1) / kTraceBlock;
+    return uint32_t(std::max<uint64_t>(1, std::min(full, need)));
+}
+
 hipError_t launch_trace_closest(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, float* t, float* b0, float* b1,
                                 int32_t* prim, bool refine) {
-    const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
+    uint32_t* next = ctx->next_counter;
+    const dim3 block(kTraceBlock);
     return timed(ctx, kKTraceClosest, [&]() {
+        hipLaunchKernelGGL(k_zero_next, dim3(1), dim3(64), 0, ctx->stream, next);
         if (refine)
-            hipLaunchKernelGGL(k_trace_closest<true>, grid, block, 0, ctx->stream, ctx->nodes.ptr, ctx->tris.ptr,
-                               soa_of(rays), n, t, b0, b1, prim);
+            hipLaunchKernelGGL(k_trace_closest<true>, dim3(persistent_blocks(ctx, k_trace_closest<true>, n)), block, 0,
+                               ctx->stream, ctx->nodes.ptr, ctx->tris.ptr, soa_of(rays), n, next, t, b0, b1, prim);
         else
-            hipLaunchKernelGGL(k_trace_closest<false>, grid, block, 0, ctx->stream, ctx->nodes.ptr, ctx->tris.ptr,
-                               soa_of(rays), n, t, b0, b1, prim);
+            hipLaunchKernelGGL(k_trace_closest<false>, dim3(persistent_blocks(ctx, k_trace_closest<false>, n)), block,
+                               0, ctx->stream, ctx->nodes.ptr, ctx->tris.ptr, soa_of(rays), n, next, t, b0, b1, prim);
     });
 }
 
 hipError_t launch_trace_any(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, uint8_t* occluded) {
-    const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
+    uint32_t* next = ctx->next_counter;
     return timed(ctx, kKTraceAny, [&]() {
-        hipLaunchKernelGGL(k_trace_any, grid, block, 0, ctx->stream, ctx->nodes.ptr, ctx->tris.ptr, soa_of(rays), n,
-                           occluded);
+        hipLaunchKernelGGL(k_zero_next, dim3(1), dim3(64), 0, ctx->stream, next);
+        hipLaunchKernelGGL(k_trace_any, dim3(persistent_blocks(ctx, k_trace_any, n)), dim3(kTraceBlock), 0,
+                           ctx->stream, ctx->nodes.ptr, ctx->tris.ptr, soa_of(rays), n, next, occluded);
     });
 }
 

@@ -127,30 +127,6 @@ __device__ __forceinline__ void refine_fp64(const RayIn& r, float4 v0, float4 v1
     b1 = float(v);
 }
 
-struct Stack {
-    uint32_t* lds;  // this block's LDS stack base
-    int lane;       // thread index in the block
-    int sp;
-    bool overflow;
-    uint32_t priv[kStackPriv];
-
-    __device__ __forceinline__ void push(uint32_t v) {
-        if (sp < kStackLDS) {
-            lds[sp * kTraceBlock + lane] = v;
-        } else if (sp < kStackTotal) {
-            priv[sp - kStackLDS] = v;
-        } else {
-            overflow = true;  // unreachable for trees built with max_depth * 3 <= kStackTotal
-            return;
-        }
-        sp++;
-    }
-    __device__ __forceinline__ uint32_t pop() {
-        sp--;
-        return sp < kStackLDS ? lds[sp * kTraceBlock + lane] : priv[sp - kStackLDS];
-    }
-};
-
 constexpr float kBoxRobust = 1.0f + 2.0f * 1.8e-7f;  // 1 + 2*gamma(3), gamma(n) = n*u/(1-n*u), u = 2^-24
 
 __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
@@ -163,14 +139,121 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32
     va = tv;
 }
 
-// Core traversal.  kAny: stop at the first accepted triangle.  Returns true on a hit;
-// for closest hit, h.t/h.rec/h.prim describe the nearest triangle.
+constexpr uint32_t kDone = 0xffffffffu;
+
+// Slab tests of the four children of inner node `cur`.  Returns how many the ray enters
+// within [tmin, tmax]; with kSort the hit children come first in (key, val), nearest first.
+template <bool kSort>
+__device__ __forceinline__ int node_children(const Node4* __restrict__ nodes, uint32_t cur, const RayIn& r, float ix,
+                                             float iy, float iz, float tmin, float tmax, float key[4],
+                                             uint32_t val[4]) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+    const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+    const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+    int nhit = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float t0x = (lx[i] - r.ox) * ix, t1x = (hx[i] - r.ox) * ix;
+        const float t0y = (ly[i] - r.oy) * iy, t1y = (hy[i] - r.oy) * iy;
+        const float t0z = (lz[i] - r.oz) * iz, t1z = (hz[i] - r.oz) * iz;
+        const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+        const bool hit = c[i] != kEmptyChild && tn <= tf * kBoxRobust;
+        key[i] = hit ? tn : INFINITY;
+        val[i] = c[i];
+        nhit += hit ? 1 : 0;
+    }
+    if (kSort) {
+        cswap(key[0], val[0], key[1], val[1]);
+        cswap(key[2], val[2], key[3], val[3]);
+        cswap(key[0], val[0], key[2], val[2]);
+        cswap(key[1], val[1], key[3], val[3]);
+        cswap(key[1], val[1], key[2], val[2]);
+    }
+    return nhit;
+}
+
+// Triangles of leaf reference `cur`.  Accepted hits shrink tmax and update h; with kAny the
+// test stops at the first accepted triangle.  Returns whether any triangle was accepted.
+template <bool kAny, bool kCount>
+__device__ __forceinline__ bool leaf_hits(const TriRecord* __restrict__ tris, uint32_t cur, const Shear& sh,
+                                          float tmin, float& tmax, HitOut& h, TraceCounters& cnt) {
+    const uint32_t first = cur & kMaxLeafFirst;
+    const uint32_t count = ((cur >> 27) & 0xfu) + 1u;
+    bool any = false;
+    for (uint32_t j = 0; j < count; j++) {
+        if (kCount) cnt.tris++;
+        const float4* tp = reinterpret_cast<const float4*>(tris + first + j);
+        const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+        float t, U, V, W, det;
+        if (watertight(sh, v0, v1, v2, tmin, tmax, t, U, V, W, det)) {
+            any = true;
+            tmax = t;
+            h.t = t;
+            h.rec = first + j;
+            h.prim = __float_as_uint(v0.w);
+            if (kAny) break;
+        }
+    }
+    return any;
+}
+
+__device__ __forceinline__ float safe_rcp(float d) { return 1.f / (fabsf(d) > 1e-30f ? d : copysignf(1e-30f, d)); }
+
+// The per-lane traversal stack: levels [0, kStackLDS) live in LDS (`lds` is this lane's
+// column of the block's __shared__ array, lds[level * kTraceBlock]), deeper levels in
+// private memory.  The builder bounds 3 * depth <= kStackTotal, so the guard never fires.
+// The spill levels are 64-bit slots so the compiler cannot merge the two loads of pop()
+// into one generic (flat) load through a selected pointer.
+#define LMGPU_TRACE_STACK(lds)                                                  \
+    uint64_t spill[kStackPriv];                                                 \
+    int sp = 0;                                                                 \
+    auto push = [&](uint32_t v) {                                               \
+        if (sp < kStackLDS) (lds)[sp * kTraceBlock] = v;                        \
+        else if (sp < kStackTotal) spill[sp - kStackLDS] = v;                   \
+        else return;                                                            \
+        sp++;                                                                   \
+    };                                                                          \
+    auto pop = [&]() -> uint32_t {                                              \
+        sp--;                                                                   \
+        if (sp < kStackLDS) return (lds)[sp * kTraceBlock];                     \
+        return uint32_t(spill[sp - kStackLDS]);                                 \
+    };
+
+// Descends from the node on top: push the farther hit children (or all but one for any
+// hit) and return the next reference to visit.
+template <bool kAny, class Push, class Pop>
+__device__ __forceinline__ uint32_t descend(int nhit, const float key[4], const uint32_t val[4], int sp, Push& push,
+                                            Pop& pop) {
+    if (nhit == 0) return sp > 0 ? pop() : kDone;
+    if (!kAny) {
+        if (nhit > 3) push(val[3]);
+        if (nhit > 2) push(val[2]);
+        if (nhit > 1) push(val[1]);
+        return val[0];
+    }
+    uint32_t first = kDone;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (key[i] != INFINITY) {
+            if (first == kDone) first = val[i];
+            else push(val[i]);
+        }
+    }
+    return first;
+}
+
+// Core traversal of one ray.  kAny: stop at the first accepted triangle.  Returns true on
+// a hit; for closest hit, h.t/h.rec/h.prim describe the nearest triangle.
 template <bool kAny, bool kCount>
 __device__ __forceinline__ bool traverse(const Node4* __restrict__ nodes, const TriRecord* __restrict__ tris,
-                                         const RayIn& r, Stack& st, HitOut& h, TraceCounters& cnt) {
-    const float ix = 1.f / (fabsf(r.dx) > 1e-30f ? r.dx : copysignf(1e-30f, r.dx));
-    const float iy = 1.f / (fabsf(r.dy) > 1e-30f ? r.dy : copysignf(1e-30f, r.dy));
-    const float iz = 1.f / (fabsf(r.dz) > 1e-30f ? r.dz : copysignf(1e-30f, r.dz));
+                                         const RayIn& r, uint32_t* lds, HitOut& h, TraceCounters& cnt) {
+    LMGPU_TRACE_STACK(lds)
+    const float ix = safe_rcp(r.dx), iy = safe_rcp(r.dy), iz = safe_rcp(r.dz);
     const Shear sh = make_shear(r);
     float tmax = r.tmax;
     const float tmin = r.tmin;
@@ -178,85 +261,129 @@ __device__ __forceinline__ bool traverse(const Node4* __restrict__ nodes, const 
     h.prim = 0xffffffffu;
     h.rec = 0;
     h.t = tmax;
-
-    st.sp = 0;
     uint32_t cur = 0;  // root node
-    constexpr uint32_t kDone = 0xffffffffu;
     while (cur != kDone) {
         // Inner nodes: descend until a leaf reference is current, or the stack runs out.
         while (cur != kDone && !(cur & kLeafBit)) {
             if (kCount) cnt.nodes++;
-            const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-            const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-            const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-            const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-            const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-            const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-            const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
             float key[4];
             uint32_t val[4];
-            int nhit = 0;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float t0x = (lx[i] - r.ox) * ix, t1x = (hx[i] - r.ox) * ix;
-                const float t0y = (ly[i] - r.oy) * iy, t1y = (hy[i] - r.oy) * iy;
-                const float t0z = (lz[i] - r.oz) * iz, t1z = (hz[i] - r.oz) * iz;
-                const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-                const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
-                const bool hit = c[i] != kEmptyChild && tn <= tf * kBoxRobust;
-                key[i] = hit ? tn : INFINITY;
-                val[i] = c[i];
-                nhit += hit ? 1 : 0;
-            }
-            if (nhit == 0) {
-                cur = st.sp > 0 ? st.pop() : kDone;
-                continue;
-            }
-            if (!kAny) {
-                cswap(key[0], val[0], key[1], val[1]);
-                cswap(key[2], val[2], key[3], val[3]);
-                cswap(key[0], val[0], key[2], val[2]);
-                cswap(key[1], val[1], key[3], val[3]);
-                cswap(key[1], val[1], key[2], val[2]);
-                if (nhit > 3) st.push(val[3]);
-                if (nhit > 2) st.push(val[2]);
-                if (nhit > 1) st.push(val[1]);
-                cur = val[0];
-            } else {
-                // Any order will do: compact the hit children, keep the first.
-                uint32_t first = kDone;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    if (key[i] != INFINITY) {
-                        if (first == kDone) first = val[i];
-                        else st.push(val[i]);
-                    }
-                }
-                cur = first;
-            }
+            const int nhit = node_children<!kAny>(nodes, cur, r, ix, iy, iz, tmin, tmax, key, val);
+            cur = descend<kAny>(nhit, key, val, sp, push, pop);
         }
         // Leaves: test their triangles, then continue with the next stack entry.
         while (cur != kDone && (cur & kLeafBit)) {
-            const uint32_t first = cur & kMaxLeafFirst;
-            const uint32_t count = ((cur >> 27) & 0xfu) + 1u;
-            for (uint32_t j = 0; j < count; j++) {
-                if (kCount) cnt.tris++;
-                const float4* tp = reinterpret_cast<const float4*>(tris + first + j);
-                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
-                float t, U, V, W, det;
-                if (watertight(sh, v0, v1, v2, tmin, tmax, t, U, V, W, det)) {
-                    found = true;
-                    tmax = t;
-                    h.t = t;
-                    h.rec = first + j;
-                    h.prim = __float_as_uint(v0.w);
-                    if (kAny) return true;
-                }
+            if (leaf_hits<kAny, kCount>(tris, cur, sh, tmin, tmax, h, cnt)) {
+                found = true;
+                if (kAny) return true;
             }
-            cur = st.sp > 0 ? st.pop() : kDone;
+            cur = sp > 0 ? pop() : kDone;
         }
     }
     return found;
+}
+
+// Lanes of a persistent traversal fetch new rays once this many lanes of the wave are idle.
+constexpr int kRefillIdle = 16;
+
+// Persistent traversal with per-lane ray refill (Aila & Laine, "Understanding the
+// efficiency of ray traversal on GPUs", HPG 2009): every wave of the grid loops over
+// the batch.  The first 64 rays of a wave are assigned statically (wave w takes
+// [64 w, 64 w + 64)); after that a lane whose ray has finished fetches the next unclaimed
+// ray, with one atomic per wave on `next` (counting from 64 x the number of waves), once
+// kRefillIdle lanes are idle.  A batch smaller than the grid therefore costs no atomics.
+// `src(i)` returns ray i, `sink(i, found, h, r)` consumes its result.  Every wave exits
+// once the batch is exhausted and its own lanes are done.
+template <bool kAny, class Source, class Sink>
+__device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes, const TriRecord* __restrict__ tris,
+                                                 uint32_t n, uint32_t* next, uint32_t* lds, Source& src, Sink& sink) {
+    LMGPU_TRACE_STACK(lds)
+    constexpr uint32_t kNone = 0xffffffffu;
+    const uint32_t lane = __lane_id();
+    uint32_t id = kNone;
+    RayIn r{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f};
+    float ix = 0.f, iy = 0.f, iz = 0.f, tmax = 0.f;
+    Shear sh = make_shear(r);
+    HitOut h{0.f, 0u, kNone};
+    bool found = false;
+    uint32_t cur = kDone;
+    TraceCounters cnt{0, 0};
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t static_total = waves * 64u;
+    {
+        const uint32_t k = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u + lane;
+        if (k < n) {
+            id = k;
+            r = src(k);
+            ix = safe_rcp(r.dx);
+            iy = safe_rcp(r.dy);
+            iz = safe_rcp(r.dz);
+            sh = make_shear(r);
+            tmax = r.tmax;
+            h.t = tmax;
+            cur = 0;
+        }
+    }
+    bool exhausted = static_total >= n;  // wave-uniform
+    bool first = true;
+    for (;;) {
+        if (!exhausted && !first) {
+            const bool need = id == kNone;
+            const unsigned long long mask = __ballot(need);
+            const uint32_t cnt_need = uint32_t(__popcll(mask));
+            if (cnt_need > 0) {
+                const int leader = __ffsll((long long)mask) - 1;
+                uint32_t base = 0;
+                if (int(lane) == leader) base = atomicAdd(next, cnt_need);
+                base = __shfl(base, leader, 64) + static_total;
+                if (base + cnt_need >= n) exhausted = true;
+                if (need) {
+                    const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+                    if (k < n) {
+                        id = k;
+                        r = src(k);
+                        ix = safe_rcp(r.dx);
+                        iy = safe_rcp(r.dy);
+                        iz = safe_rcp(r.dz);
+                        sh = make_shear(r);
+                        tmax = r.tmax;
+                        h.prim = kNone;
+                        h.rec = 0;
+                        h.t = tmax;
+                        found = false;
+                        cur = 0;
+                        sp = 0;
+                    }
+                }
+            }
+        }
+        first = false;
+        if (__ballot(id != kNone) == 0ull) break;
+        for (;;) {
+            while (cur != kDone && !(cur & kLeafBit)) {
+                float key[4];
+                uint32_t val[4];
+                const int nhit = node_children<!kAny>(nodes, cur, r, ix, iy, iz, r.tmin, tmax, key, val);
+                cur = descend<kAny>(nhit, key, val, sp, push, pop);
+            }
+            while (cur != kDone && (cur & kLeafBit)) {
+                if (leaf_hits<kAny, false>(tris, cur, sh, r.tmin, tmax, h, cnt)) {
+                    found = true;
+                    if (kAny) {
+                        cur = kDone;
+                        break;
+                    }
+                }
+                cur = sp > 0 ? pop() : kDone;
+            }
+            if (id != kNone && cur == kDone) {
+                sink(id, found, h, r);
+                id = kNone;
+            }
+            const uint32_t idle = uint32_t(__popcll(__ballot(id == kNone)));
+            if (idle == 64u || (!exhausted && idle >= uint32_t(kRefillIdle))) break;
+        }
+    }
 }
 
 // Barycentrics of the recorded winner (weights of v1 and v2), FP32 or FP64.
