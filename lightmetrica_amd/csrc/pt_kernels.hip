@@ -523,7 +523,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict_
     const uint32_t n = *count;
     QueueSource src{q};
     HitSink sink{tris, hits};
-    trace_persistent<false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink);
+    TraceCounters cnt{0, 0};
+    trace_persistent<false, false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
 }
 
 // pt: pathtrace.cpp:203-262 for one segment.
@@ -668,7 +669,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue 
     const uint32_t n = *count;
     ShadowSource src{sq};
     ShadowSink sink{sq, c.film, c.film_fp64};
-    trace_persistent<true>(c.nodes, c.tris, n, next, lds_stack + threadIdx.x, src, sink);
+    TraceCounters cnt{0, 0};
+    trace_persistent<true, false>(c.nodes, c.tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {

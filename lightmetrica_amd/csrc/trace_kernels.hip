@@ -60,7 +60,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(const Node4* __re
     __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
     SoASource src{rays};
     ClosestSink<kRefine> sink{tris, t_out, b0_out, b1_out, prim_out};
-    trace_persistent<false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink);
+    TraceCounters cnt{0, 0};
+    trace_persistent<false, false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_any(const Node4* __restrict__ nodes,
@@ -69,24 +70,28 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_any(const Node4* __restri
     __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
     SoASource src{rays};
     OcclusionSink sink{occluded};
-    trace_persistent<true>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink);
+    TraceCounters cnt{0, 0};
+    trace_persistent<true, false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
 }
 
 __global__ void k_zero_next(uint32_t* p) {
     if (threadIdx.x == 0) *p = 0u;
 }
 
+struct NullSink {
+    __device__ __forceinline__ void operator()(uint32_t, bool, const HitOut&, const RayIn&) const {}
+};
+
+// The closest-hit kernel's traversal with counters: node visits and triangle tests.
 __global__ __launch_bounds__(kTraceBlock) void k_trace_count(const Node4* __restrict__ nodes,
                                                              const TriRecord* __restrict__ tris, RaysSoA rays,
-                                                             uint32_t n, unsigned long long* __restrict__ counts) {
+                                                             uint32_t n, uint32_t* next,
+                                                             unsigned long long* __restrict__ counts) {
     __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    SoASource src{rays};
+    NullSink sink;
     TraceCounters cnt{0, 0};
-    if (i < n) {
-        const RayIn r = load_ray(rays, i);
-        HitOut h;
-        traverse<false, true>(nodes, tris, r, lds_stack + threadIdx.x, h, cnt);
-    }
+    trace_persistent<false, true>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
     // Wave-level reduction, one atomic per wave per counter.
     unsigned long long a = cnt.nodes, b = cnt.tris;
     for (int off = 32; off > 0; off >>= 1) {
@@ -136,9 +141,10 @@ hipError_t launch_trace_any(lmgpu_ctx* ctx, const float* const* rays, uint32_t n
 }
 
 hipError_t launch_trace_count(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, unsigned long long* counts) {
-    const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
-    hipLaunchKernelGGL(k_trace_count, grid, block, 0, ctx->stream, ctx->nodes.ptr, ctx->tris.ptr, soa_of(rays), n,
-                       counts);
+    uint32_t* next = ctx->next_counter;
+    hipLaunchKernelGGL(k_zero_next, dim3(1), dim3(64), 0, ctx->stream, next);
+    hipLaunchKernelGGL(k_trace_count, dim3(persistent_blocks(ctx, k_trace_count, n)), dim3(kTraceBlock), 0, ctx->stream,
+                       ctx->nodes.ptr, ctx->tris.ptr, soa_of(rays), n, next, counts);
     return hipGetLastError();
 }
 

@@ -292,11 +292,16 @@ constexpr int kRefillIdle = 16;
 // [64 w, 64 w + 64)); after that a lane whose ray has finished fetches the next unclaimed
 // ray, with one atomic per wave on `next` (counting from 64 x the number of waves), once
 // kRefillIdle lanes are idle.  A batch smaller than the grid therefore costs no atomics.
+// Leaves are processed speculatively (same paper): a lane that reaches a leaf postpones it
+// and keeps descending until every lane still descending holds a leaf, so the inner-node
+// and triangle loops each run with as many lanes as possible.
 // `src(i)` returns ray i, `sink(i, found, h, r)` consumes its result.  Every wave exits
-// once the batch is exhausted and its own lanes are done.
-template <bool kAny, class Source, class Sink>
+// once the batch is exhausted and its own lanes are done.  With kCount, `cnt` sums the
+// lane's node visits and triangle tests over all rays it traced.
+template <bool kAny, bool kCount, class Source, class Sink>
 __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes, const TriRecord* __restrict__ tris,
-                                                 uint32_t n, uint32_t* next, uint32_t* lds, Source& src, Sink& sink) {
+                                                 uint32_t n, uint32_t* next, uint32_t* lds, Source& src, Sink& sink,
+                                                 TraceCounters& cnt) {
     LMGPU_TRACE_STACK(lds)
     constexpr uint32_t kNone = 0xffffffffu;
     const uint32_t lane = __lane_id();
@@ -306,23 +311,29 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
     Shear sh = make_shear(r);
     HitOut h{0.f, 0u, kNone};
     bool found = false;
-    uint32_t cur = kDone;
-    TraceCounters cnt{0, 0};
+    uint32_t cur = kDone;   // next node or leaf reference to visit
+    uint32_t leaf = kDone;  // postponed leaf
+    auto start = [&](uint32_t k) {
+        id = k;
+        r = src(k);
+        ix = safe_rcp(r.dx);
+        iy = safe_rcp(r.dy);
+        iz = safe_rcp(r.dz);
+        sh = make_shear(r);
+        tmax = r.tmax;
+        h.prim = kNone;
+        h.rec = 0;
+        h.t = tmax;
+        found = false;
+        cur = 0;
+        leaf = kDone;
+        sp = 0;
+    };
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t static_total = waves * 64u;
     {
         const uint32_t k = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u + lane;
-        if (k < n) {
-            id = k;
-            r = src(k);
-            ix = safe_rcp(r.dx);
-            iy = safe_rcp(r.dy);
-            iz = safe_rcp(r.dz);
-            sh = make_shear(r);
-            tmax = r.tmax;
-            h.t = tmax;
-            cur = 0;
-        }
+        if (k < n) start(k);
     }
     bool exhausted = static_total >= n;  // wave-uniform
     bool first = true;
@@ -339,42 +350,41 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
                 if (base + cnt_need >= n) exhausted = true;
                 if (need) {
                     const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-                    if (k < n) {
-                        id = k;
-                        r = src(k);
-                        ix = safe_rcp(r.dx);
-                        iy = safe_rcp(r.dy);
-                        iz = safe_rcp(r.dz);
-                        sh = make_shear(r);
-                        tmax = r.tmax;
-                        h.prim = kNone;
-                        h.rec = 0;
-                        h.t = tmax;
-                        found = false;
-                        cur = 0;
-                        sp = 0;
-                    }
+                    if (k < n) start(k);
                 }
             }
         }
         first = false;
         if (__ballot(id != kNone) == 0ull) break;
         for (;;) {
+            // Inner nodes, until every lane still descending holds a postponed leaf.
             while (cur != kDone && !(cur & kLeafBit)) {
+                if (kCount) cnt.nodes++;
                 float key[4];
                 uint32_t val[4];
                 const int nhit = node_children<!kAny>(nodes, cur, r, ix, iy, iz, r.tmin, tmax, key, val);
                 cur = descend<kAny>(nhit, key, val, sp, push, pop);
+                if (cur != kDone && (cur & kLeafBit) && leaf == kDone) {
+                    leaf = cur;
+                    cur = sp > 0 ? pop() : kDone;
+                }
+                if (__ballot(leaf == kDone) == 0ull) break;
             }
-            while (cur != kDone && (cur & kLeafBit)) {
-                if (leaf_hits<kAny, false>(tris, cur, sh, r.tmin, tmax, h, cnt)) {
+            // Postponed leaves, then any leaf that is next in line.
+            while (leaf != kDone) {
+                if (leaf_hits<kAny, kCount>(tris, leaf, sh, r.tmin, tmax, h, cnt)) {
                     found = true;
                     if (kAny) {
                         cur = kDone;
+                        leaf = kDone;
                         break;
                     }
                 }
-                cur = sp > 0 ? pop() : kDone;
+                leaf = kDone;
+                if (cur != kDone && (cur & kLeafBit)) {
+                    leaf = cur;
+                    cur = sp > 0 ? pop() : kDone;
+                }
             }
             if (id != kNone && cur == kDone) {
                 sink(id, found, h, r);
