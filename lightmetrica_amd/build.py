@@ -71,13 +71,31 @@ def build_lib(jobs: int = 4) -> Path:
     return LIB
 
 
+PLUGIN = PKG / "plugin.hip.so"
+DRIVER = ROOT / "tests" / "native" / "plugin_driver"
+
+
+def build_plugin() -> Path:
+    """plugin.hip.so: host C++ over the C ABI, linked against liblmgpu.so next to it."""
+    tmp = PLUGIN.with_suffix(".so.tmp")
+    _run(["g++", *HOST_FLAGS, "-shared", str(CSRC / "plugin" / "plugin_hip.cpp"), "-o", str(tmp),
+          "-L", str(PKG), "-llmgpu", "-Wl,-rpath,$ORIGIN", "-ldl"])
+    os.replace(tmp, PLUGIN)
+    return PLUGIN
+
+
+def build_driver() -> Path:
+    """The C++ host that loads plugins from a directory like the reference's tests do."""
+    tmp = DRIVER.with_name(DRIVER.name + ".tmp")
+    _run(["g++", *HOST_FLAGS, str(ROOT / "tests" / "native" / "plugin_driver.cpp"), "-o", str(tmp), "-ldl"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build_all(jobs: int = 4) -> None:
     build_lib(jobs)
-    plugin = CSRC / "plugin" / "build_plugin.py"
-    if plugin.exists():
-        from lightmetrica_amd.csrc.plugin import build_plugin  # type: ignore
-
-        build_plugin.build(HIPCC, HOST_FLAGS, LIB)
+    build_plugin()
+    build_driver()
 
 
 if __name__ == "__main__":
