@@ -7,8 +7,11 @@ offsets N(0, 0.01)), 16M rays (origins uniform in [-0.5, 1.5]^3, directions unif
 the sphere), 4-wide SAH BVH, FP32 watertight test.  One step = one closest-hit pass
 over the 16M rays, with rays, BVH and outputs resident in HBM.
 
-Also reported (keys "path_tracing"): the room scene (256 x 256, 16 spp, max depth 8,
-RR from depth 3) rendered with the `pt` estimator -- M path-samples/s and Mrays/s.
+Also reported (key "path_tracing"): the room scene (256 x 256, 16 spp, max depth 8,
+RR from depth 3) rendered with the `pt` estimator -- M path-samples/s and Mrays/s; and
+(key "adjoint") the inverse-rendering workload -- the room plus 2,000 boxes (96k
+triangles), 512 x 512 x 32 spp, max depth 8, L2 loss against a target rendered with hidden
+box reflectances -- forward and path-replay backward times and their ratio.
 
 `roofline` is measured live: HIP events around every traversal launch on the launch
 stream give the average kernel duration; algorithmic bytes per launch come from the
@@ -159,6 +162,45 @@ def run_pt(args, Device, scenes):
     }, sc
 
 
+def run_adjoint(args, Device, scenes):
+    """Forward render and path-replay backward of the gradient workload (pt, no RR)."""
+    import numpy as np
+
+    sc = scenes.gradient_scene(512, 512, 32)
+    boxes = np.asarray(sc.meta["box_materials"])
+    albedo = np.array([m["R"] for m in sc.materials], np.float64)
+    emission = np.array([l["Le"] for l in sc.lights], np.float64)
+    hidden = albedo.copy()
+    hidden[boxes] = scenes.hidden_box_albedo(len(boxes))
+    dev = Device(0)
+    dev.upload(sc)
+    dev.build_bvh()
+    dev.set_params(hidden, emission)
+    target, _ = dev.render("pt", film_precision="fp32")
+    dev.set_params(albedo, emission)
+    fwd, bwd, res = [], [], []
+    for _ in range(args.pt_steps):
+        img, st = dev.render("pt", film_precision="fp32")
+        fwd.append(st.seconds)
+        d = (2.0 * (img - target)).astype(np.float32)
+        ga, ge, info = dev.backward("pt", d)
+        bwd.append(info["seconds"])
+        res.append(info["replay_residual"])
+    dev.close()
+    f, b = sum(fwd) / len(fwd), sum(bwd) / len(bwd)
+    return {
+        "workload": "gradient: room + 2000 axis-aligned boxes (96,042 triangles), 4 ceiling lights; 512x512, "
+                    "32 spp (8,388,608 samples), max_path_vertices 9, no RR; renderer pt; L2 loss vs a target "
+                    "with hidden box reflectances U[0.1,0.9]; parameters: 2004 materials x RGB + 4 lights x RGB",
+        "forward_ms": round(f * 1e3, 3),
+        "backward_ms": round(b * 1e3, 3),
+        "backward_over_forward": round(b / f, 3),
+        "forward_M_path_samples_per_s": round(sc.render.num_samples / f / 1e6, 1),
+        "replay_residual": max(res),
+        "gradient_norm": float(np.linalg.norm(ga)),
+    }
+
+
 def cpu_baseline_traversal(sc, rays, sample_rays):
     from oracle import lmref
 
@@ -196,8 +238,9 @@ def main():
     ap.add_argument("--pt-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pt", action="store_true")
-    ap.add_argument("--cpu-sample-rays", type=int, default=4_000_000)
-    ap.add_argument("--cpu-sample-paths", type=int, default=1_048_576)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--cpu-sample-rays", type=int, default=16_000_000)
+    ap.add_argument("--cpu-sample-paths", type=int, default=32 * 1_048_576)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -270,6 +313,8 @@ def main():
     if not args.no_pt:
         pt, room = run_pt(args, Device, scenes)
         line["path_tracing"] = pt
+    if not args.no_adjoint:
+        line["adjoint"] = run_adjoint(args, Device, scenes)
     if not args.no_cpu_baseline and rank == 0:
         line["cpu_baseline"] = cpu_baseline_traversal(sc, rays, args.cpu_sample_rays)
         if not args.no_pt:

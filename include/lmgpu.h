@@ -24,6 +24,11 @@
  *                        (src/liblightmetrica/pathtrace.cpp:165-266, pathtrace.direct.cpp:170-338,
  *                        pathtrace.mis.cpp:167-369) and HDRBitmapFilm::AccumulateContribution + Rescale
  *                        (src/liblightmetrica/hdrfilm.cpp:151-171, :334-341)
+ *   lmgpu_pt_backward    no counterpart in the reference: the path-replay adjoint of lmgpu_pt_render
+ *                        with respect to material reflectances and light emissions
+ *   lmgpu_scene_set_params  the `diffuse_reflectance` / `specular_reflectance` and `luminance` parameters
+ *                        of the assets (diffuse.cpp:73-100, mirror.cpp:47-66, light.area.cpp:71-107,
+ *                        light.env.const.cpp:60-86) changed in place, without re-uploading geometry
  *
  * Threading: a context is driven by one host thread.  All work of a context runs
  * on the context's HIP stream (its own, or the one set with lmgpu_set_stream).
@@ -40,7 +45,7 @@
 extern "C" {
 #endif
 
-#define LMGPU_ABI_VERSION 1
+#define LMGPU_ABI_VERSION 2
 
 typedef int lmgpu_status;
 #define LMGPU_OK 0
@@ -141,6 +146,9 @@ typedef struct lmgpu_hits_soa {
 #define LMGPU_PT_DIRECT 1 /* renderer "pt.direct" -- pathtrace.direct.cpp */
 #define LMGPU_PT_MIS 2    /* renderer "pt.mis"    -- pathtrace.mis.cpp */
 
+/* Film accumulation on the device.  FP32: the reference's single-precision film sums;
+ * FP64: its double-precision build's; FP32_COMPENSATED: FP32 sums plus FP32 sums of their
+ * exact rounding errors (TwoSum on the value each atomic add returns), resolved in FP64. */
 #define LMGPU_FILM_FP32 0
 #define LMGPU_FILM_FP64 1
 #define LMGPU_FILM_FP32_COMPENSATED 2
@@ -165,6 +173,18 @@ typedef struct lmgpu_film {
     int precision;
     uint32_t width, height;
 } lmgpu_film;
+
+/* Adjoint of a render: given dL/dI for the film lmgpu_pt_render returns with the same
+ * params, the gradient of L with respect to every material's R (RGB) and every light's Le
+ * (RGB), for the paths of those params' sample streams (the same paths the render traced). */
+typedef struct lmgpu_adjoint {
+    const float* d_image;   /* dL/dI: width*height*3 FP32 in the layout of lmgpu_film */
+    int image_device;       /* d_image is a device pointer */
+    double* d_albedo;       /* out: num_materials*3 */
+    double* d_emission;     /* out: num_lights*3 (may be NULL without lights) */
+    int grad_device;        /* d_albedo / d_emission are device pointers */
+    double replay_residual; /* out: sum|Q left at path ends| / sum|Q|, ~1e-16 when the replay retraced every path */
+} lmgpu_adjoint;
 
 typedef struct lmgpu_pt_stats {
     uint64_t closest_rays; /* extension rays traced */
@@ -197,6 +217,11 @@ lmgpu_status lmgpu_trace_count(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint3
 
 /* ---- path tracing ------------------------------------------------------------- */
 lmgpu_status lmgpu_pt_render(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_film* film, lmgpu_pt_stats* stats);
+/* Path-replay adjoint; film_precision is ignored (gradients reduce in FP64). */
+lmgpu_status lmgpu_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_adjoint* adjoint,
+                               lmgpu_pt_stats* stats);
+/* New reflectances (num_materials*3, host) and / or emissions (num_lights*3, host); NULL keeps them. */
+lmgpu_status lmgpu_scene_set_params(lmgpu_ctx* ctx, const float* albedo, const float* emission);
 
 /* ---- timing ------------------------------------------------------------------- */
 /* When enabled, every traversal / path-tracing kernel launch is bracketed by HIP

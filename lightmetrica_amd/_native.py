@@ -25,8 +25,8 @@ LMGPU_FILM_FP32, LMGPU_FILM_FP64, LMGPU_FILM_FP32_COMPENSATED = 0, 1, 2
 EXPORTS = [
     "lmgpu_create", "lmgpu_destroy", "lmgpu_last_error", "lmgpu_abi_version", "lmgpu_set_stream",
     "lmgpu_synchronize", "lmgpu_scene_upload", "lmgpu_bvh_build", "lmgpu_trace_closest", "lmgpu_trace_any",
-    "lmgpu_trace_count", "lmgpu_pt_render", "lmgpu_enable_timing", "lmgpu_kernel_timing", "lmgpu_reset_timing",
-    "lmgpu_inject_alloc_failures",
+    "lmgpu_trace_count", "lmgpu_pt_render", "lmgpu_pt_backward", "lmgpu_scene_set_params", "lmgpu_enable_timing",
+    "lmgpu_kernel_timing", "lmgpu_reset_timing", "lmgpu_inject_alloc_failures",
 ]
 
 
@@ -89,6 +89,11 @@ class Film(C.Structure):
                 ("height", C.c_uint32)]
 
 
+class Adjoint(C.Structure):
+    _fields_ = [("d_image", C.c_void_p), ("image_device", C.c_int), ("d_albedo", C.c_void_p),
+                ("d_emission", C.c_void_p), ("grad_device", C.c_int), ("replay_residual", C.c_double)]
+
+
 class PtStats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
                 ("seconds", C.c_double)]
@@ -104,6 +109,14 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is missing: run `python -m lightmetrica_amd.build` first")
+    # One HIP runtime per process.  PyTorch ships its own libamdhip64 (SONAME libamdhip64.so.7,
+    # loaded by file name libamdhip64.so); loading it first lets this library's
+    # libamdhip64.so.7 dependency bind to it.  In the other order the process would hold two
+    # runtimes and torch's would find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     vp = C.c_void_p
     st = C.c_int
@@ -121,6 +134,8 @@ def load() -> C.CDLL:
     lib.lmgpu_trace_count.argtypes = [vp, C.POINTER(RaysSoA), C.c_uint32, C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]
     lib.lmgpu_pt_render.argtypes = [vp, C.POINTER(PtParams), C.POINTER(Film), C.POINTER(PtStats)]
+    lib.lmgpu_pt_backward.argtypes = [vp, C.POINTER(PtParams), C.POINTER(Adjoint), C.POINTER(PtStats)]
+    lib.lmgpu_scene_set_params.argtypes = [vp, C.c_void_p, C.c_void_p]
     lib.lmgpu_enable_timing.argtypes = [vp, C.c_int]
     lib.lmgpu_kernel_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     lib.lmgpu_reset_timing.argtypes = [vp]

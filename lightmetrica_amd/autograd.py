@@ -1,0 +1,79 @@
+"""torch.autograd.Function over the C ABI's render / path-replay adjoint.
+
+    img = render(device, albedo, emission, renderer="pt")   # (H, W, 3), differentiable
+    loss = ((img - target) ** 2).sum()
+    loss.backward()                                           # albedo.grad, emission.grad
+
+`albedo` is (num_materials, 3) and `emission` (num_lights, 3), on the CPU or on the GPU the
+context drives.  The forward uploads them (lmgpu_scene_set_params) and renders; the
+backward runs lmgpu_pt_backward with the same sample streams, so the gradient is that of
+the very estimate the forward returned.  GPU tensors are passed as device pointers and
+the context is switched to torch's current stream for the call, so no host copy of the
+image or its gradient is made.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .gpu import Device
+
+__all__ = ["PTRender", "render"]
+
+
+def _params(t: torch.Tensor) -> np.ndarray:
+    return t.detach().to("cpu", torch.float32).numpy()
+
+
+class PTRender(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, albedo: torch.Tensor, emission: torch.Tensor, device: Device, renderer: str, opts: dict):
+        device.set_params(_params(albedo), _params(emission))
+        sc = device.scene
+        shape = (sc.camera.height, sc.camera.width, 3)
+        on_gpu = albedo.is_cuda
+        if on_gpu:
+            img = torch.empty(shape, dtype=torch.float32, device=albedo.device)
+            device.set_stream(torch.cuda.current_stream(albedo.device).cuda_stream)
+            try:
+                device.render(renderer, film_precision="fp64", out_precision="fp32", device_out_ptr=img.data_ptr(),
+                              **opts)
+            finally:
+                device.set_stream(None)
+        else:
+            out, _ = device.render(renderer, film_precision="fp64", out_precision="fp32", **opts)
+            img = torch.from_numpy(out)
+        ctx.device, ctx.renderer, ctx.opts = device, renderer, opts
+        ctx.shapes = (albedo.shape, emission.shape)
+        ctx.save_for_backward(albedo, emission)
+        return img
+
+    @staticmethod
+    def backward(ctx, grad_img: torch.Tensor):
+        albedo, emission = ctx.saved_tensors
+        device: Device = ctx.device
+        # The adjoint replays the paths of the parameters the forward rendered with.
+        device.set_params(_params(albedo), _params(emission))
+        g = grad_img.detach().to(torch.float32).contiguous()
+        if g.is_cuda:
+            ga = torch.empty((albedo.shape[0], 3), dtype=torch.float64, device=g.device)
+            ge = torch.empty((max(emission.shape[0], 1), 3), dtype=torch.float64, device=g.device)
+            device.set_stream(torch.cuda.current_stream(g.device).cuda_stream)
+            try:
+                device.backward(ctx.renderer, g.data_ptr(), d_albedo_ptr=ga.data_ptr(), d_emission_ptr=ge.data_ptr(),
+                                **ctx.opts)
+            finally:
+                device.set_stream(None)
+            ge = ge[: emission.shape[0]]
+        else:
+            a, e, _ = device.backward(ctx.renderer, g.numpy(), **ctx.opts)
+            ga, ge = torch.from_numpy(a), torch.from_numpy(e)
+        return (ga.to(albedo.dtype).to(albedo.device).reshape(ctx.shapes[0]),
+                ge.to(emission.dtype).to(emission.device).reshape(ctx.shapes[1]), None, None, None)
+
+
+def render(device: Device, albedo: torch.Tensor, emission: torch.Tensor, renderer: str = "pt", **opts) -> torch.Tensor:
+    """Differentiable render of the scene uploaded to `device` (see the module docstring).
+    `opts` are lmgpu render options: num_samples, max_path_vertices, rr_depth, seed,
+    sample_offset, wavefront_width."""
+    return PTRender.apply(albedo, emission, device, renderer, dict(opts))

@@ -109,6 +109,8 @@ struct lmgpu_ctx {
     lmgpu::DevBuf<lmgpu::LightDev> lights;
     lmgpu::DevBuf<lmgpu::LightTri> light_tris;
     uint32_t num_lights = 0, num_materials = 0;
+    std::vector<lmgpu::MaterialDev> host_mats;  // copies for lmgpu_scene_set_params
+    std::vector<lmgpu::LightDev> host_lights;
     lmgpu::CameraDev cam{};
     lmgpu::EnvDev env{};
 
@@ -127,6 +129,8 @@ struct lmgpu_ctx {
     size_t film_acc_bytes = 0;
     void* film_out_staging = nullptr;
     size_t film_out_staging_bytes = 0;
+    void* adj_buf = nullptr;  // adjoint: per-sample Q, gradients, staged dL/dI
+    size_t adj_buf_bytes = 0;
 
     // Timing
     bool timing = false;
@@ -167,5 +171,6 @@ hipError_t launch_trace_closest(lmgpu_ctx* ctx, const float* const* rays, uint32
 hipError_t launch_trace_any(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, uint8_t* occluded);
 hipError_t launch_trace_count(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, unsigned long long* counts);
 lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, lmgpu_pt_stats* stats);
+lmgpu_status run_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_adjoint* adj, lmgpu_pt_stats* stats);
 
 }  // namespace lmgpu

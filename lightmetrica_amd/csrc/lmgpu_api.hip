@@ -183,6 +183,7 @@ lmgpu_status lmgpu_destroy(lmgpu_ctx* ctx) {
     dev_free(ctx->pt_buf);
     dev_free(ctx->film_acc);
     dev_free(ctx->film_out_staging);
+    dev_free(ctx->adj_buf);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -369,7 +370,45 @@ lmgpu_status lmgpu_scene_upload(lmgpu_ctx* ctx, const lmgpu_scene_desc* d) {
         if (s != LMGPU_OK) return s;
     }
     if (!ctx->has_camera) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: shading scenes need a camera");
+    ctx->host_mats = std::move(mats);
+    ctx->host_lights = std::move(lights);
     ctx->has_shading = true;
+    return LMGPU_OK;
+}
+
+lmgpu_status lmgpu_scene_set_params(lmgpu_ctx* ctx, const float* albedo, const float* emission) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_shading) return fail(ctx, LMGPU_ERR_NO_SCENE, "lmgpu_scene_set_params: no shading scene");
+    (void)hipSetDevice(ctx->device);
+    for (size_t i = 0; albedo && i < ctx->host_mats.size(); i++)
+        for (int k = 0; k < 3; k++) {
+            const float v = albedo[3 * i + k];
+            if (!(v >= 0.f) || !std::isfinite(v))
+                return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_set_params: reflectance must be finite and >= 0");
+        }
+    for (size_t i = 0; emission && i < ctx->host_lights.size(); i++)
+        for (int k = 0; k < 3; k++) {
+            const float v = emission[3 * i + k];
+            if (!(v >= 0.f) || !std::isfinite(v))
+                return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_set_params: emission must be finite and >= 0");
+        }
+    hipError_t e = hipSuccess;
+    if (albedo) {
+        for (size_t i = 0; i < ctx->host_mats.size(); i++)
+            for (int k = 0; k < 3; k++) ctx->host_mats[i].R[k] = albedo[3 * i + k];
+        e = hipMemcpyAsync(ctx->mats.ptr, ctx->host_mats.data(), ctx->host_mats.size() * sizeof(MaterialDev),
+                           hipMemcpyHostToDevice, ctx->stream);
+    }
+    if (e == hipSuccess && emission) {
+        for (size_t i = 0; i < ctx->host_lights.size(); i++)
+            for (int k = 0; k < 3; k++) ctx->host_lights[i].Le[k] = emission[3 * i + k];
+        if (ctx->env.index >= 0)
+            for (int k = 0; k < 3; k++) ctx->env.Le[k] = emission[3 * size_t(ctx->env.index) + k];
+        e = hipMemcpyAsync(ctx->lights.ptr, ctx->host_lights.data(), ctx->host_lights.size() * sizeof(LightDev),
+                           hipMemcpyHostToDevice, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "lmgpu_scene_set_params");
     return LMGPU_OK;
 }
 
@@ -543,6 +582,14 @@ lmgpu_status lmgpu_pt_render(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgp
     if (!params) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: NULL params");
     (void)hipSetDevice(ctx->device);
     return run_pt(ctx, params, film, stats);
+}
+
+lmgpu_status lmgpu_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_adjoint* adjoint,
+                               lmgpu_pt_stats* stats) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    if (!params) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_backward: NULL params");
+    (void)hipSetDevice(ctx->device);
+    return run_pt_backward(ctx, params, adjoint, stats);
 }
 
 lmgpu_status lmgpu_enable_timing(lmgpu_ctx* ctx, int enable) {

@@ -14,13 +14,34 @@
 // Each path sample owns a Philox4x32-10 stream keyed by (seed, global sample index) and
 // draws from it in exactly the order the reference estimator calls Sampler::Next(), so
 // the GPU traces the same path as the CPU restatement for the same sample index.
+//
+// The same stage kernels, instantiated per mode, also run the adjoint (path replay, no
+// counterpart in the reference): for a loss L(I) of the rendered film I and its gradient
+// g = dL/dI, every path sample is traced twice more from its own stream.
+//   kAdjTotal -- Q_s = sum over the sample's contributions v (film pixel p) of g[p] * v,
+//                per colour channel, into a per-sample FP64 slot;
+//   kAdjGrad  -- the replay subtracts each contribution from Q_s as it is made, so at a
+//                surface vertex Q_s holds exactly the contributions made after it, each of
+//                which carries the vertex's reflectance R once as a factor (BSDF weight,
+//                or the BSDF value of the vertex's own next-event connection).  Hence
+//                dL/dR[m] += Q_s / R[m] at every vertex of material m, and
+//                dL/dLe[l] += g[p] * v / Le[l] for every contribution lit by light l.
+// Russian-roulette probabilities are held constant (the estimator stays unbiased for any
+// probability), MIS weights and all sampling pdfs do not depend on R or Le, so the
+// replay is exact for the path it retraces.  Gradients are scattered with FP64 atomics.
 #include "context.hpp"
 #include "traverse.hpp"
 
 #include <algorithm>
 #include <chrono>
 
+// Path state arithmetic of this file is not contracted into FMAs, so the render and both
+// adjoint passes (different instantiations) compute bit-identical paths.
+#pragma clang fp contract(off)
+
 namespace lmgpu {
+
+enum PTMode : int { kRender = 0, kAdjTotal = 1, kAdjGrad = 2 };
 
 constexpr int kPTBlock = 256;
 constexpr uint32_t kMiss = 0xffffffffu;
@@ -43,7 +64,12 @@ struct PTConsts {
     CameraDev cam;
     EnvDev env;
     void* film;
-    int film_fp64;
+    int film_mode;       // LMGPU_FILM_*
+    float* film_comp;    // FP32_COMPENSATED: the rounding-error sums beside the FP32 sums
+    const float* dLdI;   // adjoint: dL/dI, W*H*3
+    double* qacc;        // adjoint: per batch sample, 4 doubles (Q.xyz)
+    double* grad_mat;    // adjoint: num_materials * 3
+    double* grad_light;  // adjoint: num_lights * 3
     uint64_t seed;
     uint64_t base;  // global index of the batch's first sample
     uint32_t batch;  // samples in this batch
@@ -62,9 +88,11 @@ struct Queue {
 };
 
 struct ShadowQueue {
-    float4* o;  // origin, tmax
-    float4* d;  // direction, pixel (bits)
-    float4* c;  // contribution
+    float4* o;      // origin, tmax
+    float4* d;      // direction, pixel (bits)
+    float4* c;      // contribution without the light's Le, light index (bits)
+    uint32_t* sid;  // sample index in the batch
+    uint32_t* vis;  // adjoint passes: 1 = unoccluded
 };
 
 struct PTBuffers {
@@ -79,18 +107,6 @@ struct PTBuffers {
 constexpr int kMaxSegments = 64;
 
 // ---------------------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    const uint32_t cnt = uint32_t(__popcll(mask));
-    const uint32_t lane = __lane_id();
-    const uint32_t before = uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-    const int leader = cnt ? (__ffsll((long long)mask) - 1) : 0;
-    uint32_t base = 0;
-    if (cnt && int(lane) == leader) base = atomicAdd(counter, cnt);
-    base = __shfl(base, leader, 64);
-    return base + before;
-}
 
 // Block-aggregated queue append: one atomic per block of kPTBlock threads instead of one
 // per wavefront (same-address atomics serialise in L2).  Every thread of the block must call
@@ -116,34 +132,110 @@ __device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred) {
     return base + wave_off[wave] + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
-__device__ __forceinline__ void film_add_raw(void* film, int fp64, uint32_t pixel, V3 v) {
-    if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return;
-    if (fp64) {
-        double* f = static_cast<double*>(film) + 3ull * pixel;
-        unsafeAtomicAdd(f + 0, double(v.x));
-        unsafeAtomicAdd(f + 1, double(v.y));
-        unsafeAtomicAdd(f + 2, double(v.z));
-    } else {
-        float* f = static_cast<float*>(film) + 3ull * pixel;
-        unsafeAtomicAdd(f + 0, v.x);
-        unsafeAtomicAdd(f + 1, v.y);
-        unsafeAtomicAdd(f + 2, v.z);
-    }
+// Compensated FP32 accumulation: the atomic returns the sum it replaced, so the rounding
+// error of the stored sum is recovered exactly by TwoSum (Knuth, TAOCP vol. 2, 4.2.2) and
+// accumulated in a second FP32 array; the film value is sum + error.
+__device__ __forceinline__ void comp_add(float* sum, float* err, float x) {
+    if (x == 0.f) return;
+    const float old = unsafeAtomicAdd(sum, x);
+    const float s = old + x;  // the value the atomic stored (round to nearest even)
+    const float bp = s - old;
+    const float e = (old - (s - bp)) + (x - bp);
+    if (e != 0.f) unsafeAtomicAdd(err, e);
 }
 
 __device__ __forceinline__ void film_add(const PTConsts& c, uint32_t pixel, V3 v) {
     if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return;
-    if (c.film_fp64) {
+    if (c.film_mode == LMGPU_FILM_FP64) {
         double* f = static_cast<double*>(c.film) + 3ull * pixel;
         unsafeAtomicAdd(f + 0, double(v.x));
         unsafeAtomicAdd(f + 1, double(v.y));
         unsafeAtomicAdd(f + 2, double(v.z));
+    } else if (c.film_mode == LMGPU_FILM_FP32_COMPENSATED) {
+        float* f = static_cast<float*>(c.film) + 3ull * pixel;
+        float* e = c.film_comp + 3ull * pixel;
+        comp_add(f + 0, e + 0, v.x);
+        comp_add(f + 1, e + 1, v.y);
+        comp_add(f + 2, e + 2, v.z);
     } else {
         float* f = static_cast<float*>(c.film) + 3ull * pixel;
         unsafeAtomicAdd(f + 0, v.x);
         unsafeAtomicAdd(f + 1, v.y);
         unsafeAtomicAdd(f + 2, v.z);
     }
+}
+
+// One contribution v of batch sample `sid` to film pixel `pixel`: the film in the render,
+// +/- g[pixel] * v into the sample's Q in the adjoint passes.  Per launch a sample makes at
+// most one call, so the read-modify-write of its Q slot needs no atomics.
+template <int kMode>
+__device__ __forceinline__ void contribute(const PTConsts& c, uint32_t sid, uint32_t pixel, V3 v) {
+    if (kMode == kRender) {
+        film_add(c, pixel, v);
+        return;
+    }
+    if (is_zero(v)) return;
+    const float* g = c.dLdI + 3ull * pixel;
+    double* q = c.qacc + 4ull * sid;
+    const double x = double(g[0]) * double(v.x), y = double(g[1]) * double(v.y), z = double(g[2]) * double(v.z);
+    if (kMode == kAdjTotal) {
+        q[0] += x;
+        q[1] += y;
+        q[2] += z;
+    } else {
+        q[0] -= x;
+        q[1] -= y;
+        q[2] -= z;
+    }
+}
+
+// g[3 id + k] += (x, y, z)[k] for the active lanes.  Lanes holding the same id as the
+// first active lane are summed in registers first (a few rounds catch the walls and
+// lights most lanes share), the rest add on their own.  Every lane of the wave must call
+// it together.
+constexpr int kAggRounds = 3;
+__device__ __forceinline__ void scatter_grad(double* g, uint32_t id, double x, double y, double z, bool active) {
+#pragma unroll 1
+    for (int round = 0; round < kAggRounds; round++) {
+        const unsigned long long m = __ballot(active);
+        if (m == 0ull) return;
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t lid = __shfl(id, leader, 64);
+        const bool mine = active && id == lid;
+        double sx = mine ? x : 0.0, sy = mine ? y : 0.0, sz = mine ? z : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            sx += __shfl_xor(sx, off, 64);
+            sy += __shfl_xor(sy, off, 64);
+            sz += __shfl_xor(sz, off, 64);
+        }
+        if (int(__lane_id()) == leader) {
+            double* p = g + 3ull * lid;
+            if (sx != 0.0) unsafeAtomicAdd(p + 0, sx);
+            if (sy != 0.0) unsafeAtomicAdd(p + 1, sy);
+            if (sz != 0.0) unsafeAtomicAdd(p + 2, sz);
+        }
+        active = active && !mine;
+    }
+    if (active) {
+        double* p = g + 3ull * id;
+        if (x != 0.0) unsafeAtomicAdd(p + 0, x);
+        if (y != 0.0) unsafeAtomicAdd(p + 1, y);
+        if (z != 0.0) unsafeAtomicAdd(p + 2, z);
+    }
+}
+
+// dL/dLe of a contribution v = w * Le (w: the contribution without Le), scattered to
+// light `li`.  Wave-uniform call.
+__device__ __forceinline__ void grad_emission(const PTConsts& c, uint32_t pixel, V3 w, int li, bool active) {
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (active) {
+        const float* g = c.dLdI + 3ull * pixel;
+        x = double(g[0]) * double(w.x);
+        y = double(g[1]) * double(w.y);
+        z = double(g[2]) * double(w.z);
+    }
+    scatter_grad(c.grad_light, uint32_t(active ? li : 0), x, y, z, active);
 }
 
 // HDRBitmapFilm pixel of a raster position (hdrfilm.cpp:160-166); y = 0 is the bottom row.
@@ -319,24 +411,27 @@ struct ShadowRec {
     float4 o, d, c;
 };
 
-// Returns the shadow-ray record of the connection (emit = false when the contribution is 0;
-// the caller appends emitted records to the shadow queue).
+// Returns the shadow-ray record of the connection: its contribution without the light's Le
+// and the light index.  emit = false when the contribution is 0 (in the adjoint passes:
+// when it is 0 whatever Le is); the caller appends emitted records to the shadow queue.
 __device__ __forceinline__ ShadowRec nee(const PTConsts& c, const NEEVertex& v, V3 T, int n, uint32_t pixel,
-                                         SampleStream& rng, bool mis, bool active) {
+                                         SampleStream& rng, bool mis, bool active, bool adjoint) {
     V3 contrib = v3s(0.f);
+    V3 Le = v3s(0.f);
+    int li = 0;
     V3 target = v.p;
     uint32_t pix = pixel;
     if (active) {
         // Scene::SampleLightSelection (scene.cpp:79-86)
         float lu0 = rng.next(), lu1 = rng.next();
         const int nl = int(c.num_lights);
-        const int li = min(int(lu0 * float(nl)), nl - 1);
+        li = min(int(lu0 * float(nl)), nl - 1);
         lu0 = lu0 * float(nl) - float(li);
         const float sel_pdf = 1.f / float(nl);
         const float4 L0 = reinterpret_cast<const float4*>(c.lights)[2 * li];
         const float4 L1 = reinterpret_cast<const float4*>(c.lights)[2 * li + 1];
         const uint32_t ltype = __float_as_uint(L0.x);
-        const V3 Le = v3(L0.y, L0.z, L0.w);
+        Le = v3(L0.y, L0.z, L0.w);
         V3 lp, ln;
         float pdf_pl;
         if (ltype == LMGPU_LIGHT_ENV_CONST) {  // light.env.const.cpp:196-204
@@ -394,19 +489,19 @@ __device__ __forceinline__ ShadowRec nee(const PTConsts& c, const NEEVertex& v, 
             if (!v.camera) num *= fabsf(dot(v.s.f.sn, dv));
             num *= fabsf(dot(ln, -dv));
             const float G = fabsf(l) < kEps ? 0.f : num / l2;
-            const V3 LeP = Le * kPi;
+            // T fsE G fsL LeP / pdf with LeP = Le pi; Le is applied where the ray is resolved.
             if (!mis) {
-                contrib = T * fsE * G * fsL * LeP / pdf_pl;
+                contrib = T * fsE * G * fsL * kPi / pdf_pl;
             } else if (G != 0.f) {
                 const float pdf_dl = pdf_pl / G;
                 const float w = pdf_dl / (pdf_dl + pdf_bsdf);
-                contrib = T * w * fsE * G * fsL * LeP / pdf_pl;
+                contrib = T * w * fsE * G * fsL * kPi / pdf_pl;
             }
             target = lp;
         }
     }
     ShadowRec rec;
-    rec.emit = active && !is_zero(contrib);
+    rec.emit = active && !is_zero(contrib) && (adjoint || !is_zero(Le));
     if (rec.emit) {
         // RenderUtils::Visible: d = (p2 - p1) / |p2 - p1|, (Eps, |p2 - p1| (1 - Eps))
         const V3 dv = target - v.p;
@@ -414,15 +509,16 @@ __device__ __forceinline__ ShadowRec nee(const PTConsts& c, const NEEVertex& v, 
         const V3 d = dv / len;
         rec.o = make_float4(v.p.x, v.p.y, v.p.z, len * (1.f - kEps));
         rec.d = make_float4(d.x, d.y, d.z, __uint_as_float(pix));
-        rec.c = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
+        rec.c = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(li));
     }
     return rec;
 }
 
-__device__ __forceinline__ void put_shadow(const ShadowQueue& sq, uint32_t slot, const ShadowRec& r) {
+__device__ __forceinline__ void put_shadow(const ShadowQueue& sq, uint32_t slot, const ShadowRec& r, uint32_t sid) {
     sq.o[slot] = r.o;
     sq.d[slot] = r.d;
     sq.c[slot] = r.c;
+    sq.sid[slot] = sid;
 }
 
 // ---------------------------------------------------------------------------------
@@ -439,6 +535,7 @@ __device__ __forceinline__ void store_path(const Queue& q, uint32_t slot, V3 o, 
 
 // Camera vertex.  pt: pathtrace.cpp:167-192.  pt.direct / pt.mis: the first loop
 // iteration of pathtrace.direct.cpp:172-303 (NEE from the camera, RR, camera sampling).
+template <int kMode>
 __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_t* live_count, ShadowQueue sq,
                                                   uint32_t* shadow_count) {
     const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
@@ -468,9 +565,9 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
     v.p = cam_p;
     v.camera = true;
     v.wi = v3s(0.f);
-    const ShadowRec rec = nee(c, v, T, 1, 0u, rng, mis, alive);
+    const ShadowRec rec = nee(c, v, T, 1, 0u, rng, mis, alive, kMode != kRender);
     const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
-    if (rec.emit) put_shadow(sq, sslot, rec);
+    if (rec.emit) put_shadow(sq, sslot, rec, i);
     if (alive && c.rr_depth != -1 && 1 >= c.rr_depth) {
         const float q = fminf(0.5f, luminance(T));
         if (rng.next() > q) alive = false;
@@ -527,7 +624,21 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict_
     trace_persistent<false, false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
 }
 
+// dL/dR of the vertex's material: Q / R per channel (the contributions still in Q all carry
+// R once).  Wave-uniform call.
+__device__ __forceinline__ void grad_material(const PTConsts& c, uint32_t sid, uint32_t mat, V3 R, bool active) {
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (active) {
+        const double* q = c.qacc + 4ull * sid;
+        x = R.x != 0.f ? q[0] / double(R.x) : 0.0;
+        y = R.y != 0.f ? q[1] / double(R.y) : 0.0;
+        z = R.z != 0.f ? q[2] / double(R.z) : 0.0;
+    }
+    scatter_grad(c.grad_mat, active ? mat : 0u, x, y, z, active);
+}
+
 // pt: pathtrace.cpp:203-262 for one segment.
+template <int kMode>
 __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                        const float4* __restrict__ hits, Queue out,
                                                        uint32_t* out_count) {
@@ -541,15 +652,29 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
     int n = int(meta.y & 0xffffu);
     bool alive = false;
     Surf s;
+    s.mat = kBlackMat;
+    s.light = -1;
+    MatInfo m{LMGPU_BSDF_DIFFUSE, v3s(0.f)};
+    bool emits = false;
+    V3 Le_w = v3s(0.f);  // the emission at this hit without Le
+    V3 Le = v3s(0.f);
     V3 wo = v3s(0.f);
     float pdf = 0.f;
     SampleStream rng;
     rng.init(c.seed, c.base + meta.x, meta.z);
     if (valid && surface_at(c, o, d, ro.w, hits[j], s)) {
-        if (s.light >= 0 && dot(s.f.sn, -d) > 0.f) L = L + T * v3s(kInvPi) * (light_Le(c, s.light) * kPi);
+        if (s.light >= 0 && dot(s.f.sn, -d) > 0.f) {
+            emits = true;
+            Le = light_Le(c, s.light);
+            Le_w = T * v3s(kInvPi) * kPi;
+            const V3 v = T * v3s(kInvPi) * (Le * kPi);
+            if (kMode == kRender) L = L + v;
+            else contribute<kMode>(c, meta.x, meta.w, v);
+        }
         const float u0 = rng.next(), u1 = rng.next(), uc = rng.next();
         int type;
-        const V3 fs = bsdf_sample(mat_info(c, s.mat), s, -d, u0, u1, uc, wo, pdf, type);
+        m = mat_info(c, s.mat);
+        const V3 fs = bsdf_sample(m, s, -d, u0, u1, uc, wo, pdf, type);
         if (!is_zero(fs)) {
             T = T * fs;
             alive = true;
@@ -562,14 +687,19 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
             if (c.max_pv != -1 && n >= c.max_pv) alive = false;
         }
     }
+    if (kMode == kAdjGrad) {
+        grad_emission(c, meta.w, Le_w, s.light, emits);
+        grad_material(c, meta.x, s.mat, m.R, alive && s.mat != kBlackMat);
+    }
     const uint32_t slot = block_append<0>(out_count, alive);
     if (alive) store_path(out, slot, s.p, kEps, wo, pdf, T, L, meta.x, uint32_t(n), rng.dim, meta.w);
-    else if (valid) film_add(c, meta.w, L);
+    else if (kMode == kRender && valid) film_add(c, meta.w, L);
 }
 
 // pt.direct / pt.mis: the hit half of one loop iteration (pathtrace.direct.cpp:304-336,
 // pathtrace.mis.cpp:318-367) followed by the next iteration up to the next ray
 // (NEE :197-248 / :194-262, RR, BSDF sampling).
+template <int kMode>
 __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                         const float4* __restrict__ hits, Queue out,
                                                         uint32_t* out_count, ShadowQueue sq, uint32_t* shadow_count) {
@@ -586,12 +716,17 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     SampleStream rng;
     rng.init(c.seed, c.base + meta.x, meta.z);
     Surf s;
+    s.mat = kBlackMat;
+    s.light = -1;
     bool alive = valid && surface_at(c, o, d, ro.w, hits[j], s);
+    bool emits = false;
+    V3 Le_w = v3s(0.f);
     if (alive && s.light >= 0) {
         const float LeD = dot(s.f.sn, -d) > 0.f ? kInvPi : 0.f;
-        const V3 LeP = light_Le(c, s.light) * kPi;
+        const V3 Le = light_Le(c, s.light);
         if (flags & kFlagSpecular) {
-            film_add(c, meta.w, T * v3s(LeD) * LeP);
+            emits = true;
+            Le_w = T * v3s(LeD) * kPi;
         } else if (mis) {
             // GeneralizedGeometryTerm(previous vertex, hit) and the light-sampling pdf
             V3 dv = s.p - o;
@@ -606,9 +741,12 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
                                                          : reinterpret_cast<const float4*>(c.lights)[2 * s.light + 1].z;
             const float pdf_dl = G == 0.f ? 0.f : (1.f / float(c.num_lights)) * pos_pdf / G;
             const float w = rd.w / (rd.w + pdf_dl);
-            film_add(c, meta.w, T * v3s(LeD) * LeP * w);
+            emits = true;
+            Le_w = T * v3s(LeD) * kPi * w;
         }
+        if (emits) contribute<kMode>(c, meta.x, meta.w, Le_w * Le);
     }
+    if (kMode == kAdjGrad) grad_emission(c, meta.w, Le_w, s.light, emits);
     if (alive) {
         n++;
         if (c.max_pv != -1 && n >= c.max_pv) alive = false;
@@ -619,9 +757,9 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     v.camera = false;
     v.wi = -d;
     v.m = alive ? mat_info(c, s.mat) : MatInfo{LMGPU_BSDF_MIRROR, v3s(0.f)};
-    const ShadowRec rec = nee(c, v, T, n, meta.w, rng, mis, alive && mat_nondelta(v.m));
+    const ShadowRec rec = nee(c, v, T, n, meta.w, rng, mis, alive && mat_nondelta(v.m), kMode != kRender);
     const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
-    if (rec.emit) put_shadow(sq, sslot, rec);
+    if (rec.emit) put_shadow(sq, sslot, rec, meta.x);
     if (alive && c.rr_depth != -1 && n >= c.rr_depth) {
         const float q = fminf(0.5f, luminance(T));
         if (rng.next() > q) alive = false;
@@ -636,6 +774,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
         if (is_zero(fs)) alive = false;
         else T = T * fs;
     }
+    if (kMode == kAdjGrad) grad_material(c, meta.x, s.mat, v.m.R, (rec.emit || alive) && s.mat != kBlackMat);
     const uint32_t slot = block_append<1>(out_count, alive);
     if (alive) {
         const uint32_t nf = uint32_t(n) | ((type & (1 << 2)) ? (kFlagSpecular << 16) : 0u);
@@ -643,7 +782,9 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     }
 }
 
-// NEE shadow rays: any hit over the triangles, then the film update when unoccluded.
+// NEE shadow rays: any hit over the triangles.  The render adds the unoccluded
+// contributions to the film right away; the adjoint passes record visibility and apply
+// it in k_shadow_apply, where the wavefront is converged for the gradient scatter.
 struct ShadowSource {
     ShadowQueue sq;
     __device__ __forceinline__ RayIn operator()(uint32_t i) const {
@@ -652,25 +793,47 @@ struct ShadowSource {
     }
 };
 
+template <int kMode>
 struct ShadowSink {
+    PTConsts c;
     ShadowQueue sq;
-    void* film;
-    int fp64;
     __device__ __forceinline__ void operator()(uint32_t i, bool found, const HitOut&, const RayIn&) const {
+        if (kMode != kRender) {
+            sq.vis[i] = found ? 0u : 1u;
+            return;
+        }
         if (found) return;
         const float4 v = sq.c[i];
-        film_add_raw(film, fp64, __float_as_uint(sq.d[i].w), v3(v.x, v.y, v.z));
+        const V3 Le = light_Le(c, __float_as_int(v.w));
+        film_add(c, __float_as_uint(sq.d[i].w), v3(v.x, v.y, v.z) * Le);
     }
 };
 
+template <int kMode>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue sq, const uint32_t* __restrict__ count,
                                                         uint32_t* next) {
     __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
     const uint32_t n = *count;
     ShadowSource src{sq};
-    ShadowSink sink{sq, c.film, c.film_fp64};
+    ShadowSink<kMode> sink{c, sq};
     TraceCounters cnt{0, 0};
     trace_persistent<true, false>(c.nodes, c.tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+}
+
+// Adjoint passes: the unoccluded shadow contributions into Q, and dL/dLe.
+template <int kMode>
+__global__ __launch_bounds__(kPTBlock) void k_shadow_apply(PTConsts c, ShadowQueue sq,
+                                                           const uint32_t* __restrict__ count) {
+    const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
+    const bool valid = i < *count;
+    const uint32_t j = valid ? i : 0u;
+    const bool visible = valid && sq.vis[j] != 0u;
+    const float4 cv = sq.c[j];
+    const uint32_t pixel = __float_as_uint(sq.d[j].w);
+    const int li = __float_as_int(cv.w);
+    const V3 w = v3(cv.x, cv.y, cv.z);
+    if (visible) contribute<kMode>(c, sq.sid[j], pixel, w * light_Le(c, li));
+    if (kMode == kAdjGrad) grad_emission(c, pixel, w, li, visible);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -703,6 +866,32 @@ __global__ void k_resolve(const Acc* __restrict__ acc, Out* __restrict__ out, si
     if (i < n) out[i] = Out(acc[i] * weight);
 }
 
+// Compensated FP32 film: (sum + error) * weight, evaluated in FP64 and rounded once.
+template <class Out>
+__global__ void k_resolve_comp(const float* __restrict__ acc, const float* __restrict__ err, Out* __restrict__ out,
+                               size_t n, double weight) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = Out((double(acc[i]) + double(err[i])) * weight);
+}
+
+// sum over samples of |Q.x| + |Q.y| + |Q.z| (the replay check of the adjoint).
+__global__ __launch_bounds__(256) void k_abs_sum(const double* __restrict__ q, uint32_t n, double* out) {
+    __shared__ double part[256 / 64];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (i < n) v = fabs(q[4ull * i]) + fabs(q[4ull * i + 1]) + fabs(q[4ull * i + 2]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
+__global__ void k_scale_f64(double* p, uint32_t n, double w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] *= w;
+}
+
 // ---------------------------------------------------------------------------------
 // Host driver
 
@@ -721,128 +910,128 @@ static void carve(void* base, uint32_t cap, PTBuffers& b) {
     b.sq.o = reinterpret_cast<float4*>(p); p += size_t(cap) * 16;
     b.sq.d = reinterpret_cast<float4*>(p); p += size_t(cap) * 16;
     b.sq.c = reinterpret_cast<float4*>(p); p += size_t(cap) * 16;
+    b.sq.sid = reinterpret_cast<uint32_t*>(p); p += size_t(cap) * 4;
+    b.sq.vis = reinterpret_cast<uint32_t*>(p); p += size_t(cap) * 4;
     b.counters = reinterpret_cast<uint32_t*>(p); p += 4096;
     b.totals = reinterpret_cast<unsigned long long*>(p);
 }
 
-static size_t pt_bytes(uint32_t cap) { return 2 * queue_bytes(cap) + size_t(cap) * 16 * 4 + 4096 + 64; }
+static size_t pt_bytes(uint32_t cap) { return 2 * queue_bytes(cap) + size_t(cap) * (16 * 4 + 8) + 4096 + 64; }
 
-lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, lmgpu_pt_stats* stats) {
-    if (!ctx->has_bvh) return fail(ctx, LMGPU_ERR_NO_BVH, "lmgpu_pt_render: build the BVH first");
+// (Re)allocates a context-owned device buffer to at least `bytes`.
+static lmgpu_status ensure(lmgpu_ctx* ctx, void** buf, size_t* have, size_t bytes, const char* what) {
+    if (bytes <= *have) return LMGPU_OK;
+    dev_free(*buf);
+    *buf = nullptr;
+    *have = 0;
+    lmgpu_status s = dev_alloc(ctx, buf, bytes, what);
+    if (s != LMGPU_OK) return s;
+    *have = bytes;
+    return LMGPU_OK;
+}
+
+static lmgpu_status validate(lmgpu_ctx* ctx, const lmgpu_pt_params* p, const char* fn) {
+    const std::string f(fn);
+    if (!p) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, f + ": params is NULL");
+    if (!ctx->has_bvh) return fail(ctx, LMGPU_ERR_NO_BVH, f + ": build the BVH first");
     if (!ctx->has_shading || !ctx->has_camera)
-        return fail(ctx, LMGPU_ERR_NO_SCENE, "lmgpu_pt_render: scene has no materials or camera");
+        return fail(ctx, LMGPU_ERR_NO_SCENE, f + ": scene has no materials or camera");
     if (p->renderer < LMGPU_PT || p->renderer > LMGPU_PT_MIS)
-        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: unknown renderer");
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, f + ": unknown renderer");
     if (p->renderer != LMGPU_PT && ctx->num_lights == 0)
-        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: pt.direct / pt.mis need at least one light");
-    if (p->film_precision != LMGPU_FILM_FP32 && p->film_precision != LMGPU_FILM_FP64)
-        return fail(ctx, LMGPU_ERR_UNSUPPORTED, "lmgpu_pt_render: film precision not supported");
-    if (p->num_samples == 0) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: num_samples is 0");
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, f + ": pt.direct / pt.mis need at least one light");
+    if (p->film_precision != LMGPU_FILM_FP32 && p->film_precision != LMGPU_FILM_FP64 &&
+        p->film_precision != LMGPU_FILM_FP32_COMPENSATED)
+        return fail(ctx, LMGPU_ERR_UNSUPPORTED, f + ": film precision not supported");
+    if (p->num_samples == 0) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, f + ": num_samples is 0");
     if (p->max_path_vertices == 0 || p->max_path_vertices < -1)
-        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: max_path_vertices must be -1 or >= 1");
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, f + ": max_path_vertices must be -1 or >= 1");
     if (p->max_path_vertices == -1 || p->max_path_vertices > kMaxSegments)
-        return fail(ctx, LMGPU_ERR_UNSUPPORTED, "lmgpu_pt_render: max_path_vertices must be in [1, 64]");
-    if (!film || !film->data || film->width != uint32_t(ctx->cam.width) || film->height != uint32_t(ctx->cam.height))
-        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: film size differs from the camera's");
-    if (film->precision != LMGPU_FILM_FP32 && film->precision != LMGPU_FILM_FP64)
-        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: output film precision");
+        return fail(ctx, LMGPU_ERR_UNSUPPORTED, f + ": max_path_vertices must be in [1, 64]");
     if (3u * ctx->bvh_max_depth > uint32_t(kStackTotal))
-        return fail(ctx, LMGPU_ERR_BVH_TOO_DEEP, "lmgpu_pt_render: BVH deeper than the traversal stack");
+        return fail(ctx, LMGPU_ERR_BVH_TOO_DEEP, f + ": BVH deeper than the traversal stack");
+    return LMGPU_OK;
+}
 
-    uint32_t cap = p->wavefront_width ? p->wavefront_width : (1u << 21);
-    cap = uint32_t(std::min<uint64_t>(cap, p->num_samples));
-    cap = std::max<uint32_t>(cap, 64u);
-    const size_t need = pt_bytes(cap);
-    if (need > ctx->pt_buf_bytes) {
-        dev_free(ctx->pt_buf);
-        ctx->pt_buf = nullptr;
-        ctx->pt_buf_bytes = 0;
-        lmgpu_status s = dev_alloc(ctx, &ctx->pt_buf, need, "path queues");
-        if (s != LMGPU_OK) return s;
-        ctx->pt_buf_bytes = need;
-    }
-    ctx->pt_capacity = cap;
-    const size_t npix = size_t(ctx->cam.width) * size_t(ctx->cam.height);
-    const size_t acc_elem = p->film_precision == LMGPU_FILM_FP64 ? 8 : 4;
-    const size_t acc_bytes = (npix * 3 * acc_elem + 15) / 16 * 16;
-    if (acc_bytes > ctx->film_acc_bytes) {
-        dev_free(ctx->film_acc);
-        ctx->film_acc = nullptr;
-        ctx->film_acc_bytes = 0;
-        lmgpu_status s = dev_alloc(ctx, &ctx->film_acc, acc_bytes, "film accumulator");
-        if (s != LMGPU_OK) return s;
-        ctx->film_acc_bytes = acc_bytes;
-    }
-    const size_t out_elem = film->precision == LMGPU_FILM_FP64 ? 8 : 4;
-    const size_t out_bytes = npix * 3 * out_elem;
-    void* out_dev = film->data;
-    if (!film->device) {
-        if (out_bytes > ctx->film_out_staging_bytes) {
-            dev_free(ctx->film_out_staging);
-            ctx->film_out_staging = nullptr;
-            ctx->film_out_staging_bytes = 0;
-            lmgpu_status s = dev_alloc(ctx, &ctx->film_out_staging, out_bytes, "film staging");
-            if (s != LMGPU_OK) return s;
-            ctx->film_out_staging_bytes = out_bytes;
-        }
-        out_dev = ctx->film_out_staging;
-    }
-
-    PTBuffers b;
-    carve(ctx->pt_buf, cap, b);
+// Shared state of one render / adjoint run.
+struct Pipeline {
+    lmgpu_ctx* ctx;
     PTConsts c;
-    c.nodes = ctx->nodes.ptr;
-    c.tris = ctx->tris.ptr;
-    c.shade = ctx->shade.ptr;
-    c.mats = ctx->mats.ptr;
-    c.lights = ctx->lights.ptr;
-    c.light_tris = ctx->light_tris.ptr;
-    c.num_lights = ctx->num_lights;
-    c.cam = ctx->cam;
-    c.env = ctx->env;
-    c.film = ctx->film_acc;
-    c.film_fp64 = p->film_precision == LMGPU_FILM_FP64;
-    c.seed = p->seed;
-    c.max_pv = p->max_path_vertices;
-    c.rr_depth = p->rr_depth;
-    c.renderer = p->renderer;
+    PTBuffers b;
+    uint32_t cap;
+    uint32_t trace_grid;
+    int segments;
+    bool is_pt;
 
-    hipStream_t s = ctx->stream;
-    hipError_t e;
-    int occ_e = 0, occ_s = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_e, k_extend, kTraceBlock, 0) != hipSuccess || occ_e < 1) occ_e = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_shadow, kTraceBlock, 0) != hipSuccess || occ_s < 1) occ_s = 1;
-    const uint32_t trace_grid = uint32_t(std::max(ctx->num_cus, 1)) * uint32_t(std::min(occ_e, occ_s));
-    hipEvent_t r0, r1;
-    (void)hipEventCreate(&r0);
-    (void)hipEventCreate(&r1);
-    (void)hipEventRecord(r0, s);
-    hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((acc_bytes / 16 + 255) / 256)), dim3(256), 0, s,
-                       static_cast<uint4*>(ctx->film_acc), acc_bytes / 16);
-    hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, b.counters, (4096u + 64u) / 4u);
-    // Segments traced per path: at most max_path_vertices - 1 (pathtrace.cpp:196-199).
-    const int segments = p->max_path_vertices - 1;
-    const bool is_pt = p->renderer == LMGPU_PT;
-    for (uint64_t base = 0; base < p->num_samples; base += cap) {
-        const uint32_t nb = uint32_t(std::min<uint64_t>(cap, p->num_samples - base));
-        c.base = p->sample_offset + base;
+    lmgpu_status setup(const lmgpu_pt_params* p) {
+        cap = p->wavefront_width ? p->wavefront_width : (1u << 21);
+        cap = uint32_t(std::min<uint64_t>(cap, p->num_samples));
+        cap = std::max<uint32_t>(cap, 64u);
+        lmgpu_status st = ensure(ctx, &ctx->pt_buf, &ctx->pt_buf_bytes, pt_bytes(cap), "path queues");
+        if (st != LMGPU_OK) return st;
+        ctx->pt_capacity = cap;
+        carve(ctx->pt_buf, cap, b);
+        c = PTConsts{};
+        c.nodes = ctx->nodes.ptr;
+        c.tris = ctx->tris.ptr;
+        c.shade = ctx->shade.ptr;
+        c.mats = ctx->mats.ptr;
+        c.lights = ctx->lights.ptr;
+        c.light_tris = ctx->light_tris.ptr;
+        c.num_lights = ctx->num_lights;
+        c.cam = ctx->cam;
+        c.env = ctx->env;
+        c.seed = p->seed;
+        c.max_pv = p->max_path_vertices;
+        c.rr_depth = p->rr_depth;
+        c.renderer = p->renderer;
+        c.film_mode = p->film_precision;
+        int occ_e = 0, occ_s = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_e, k_extend, kTraceBlock, 0) != hipSuccess || occ_e < 1)
+            occ_e = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_shadow<kRender>, kTraceBlock, 0) != hipSuccess ||
+            occ_s < 1)
+            occ_s = 1;
+        trace_grid = uint32_t(std::max(ctx->num_cus, 1)) * uint32_t(std::min(occ_e, occ_s));
+        // Segments traced per path: at most max_path_vertices - 1 (pathtrace.cpp:196-199).
+        segments = p->max_path_vertices - 1;
+        is_pt = p->renderer == LMGPU_PT;
+        return LMGPU_OK;
+    }
+
+    // One batch [base, base + nb) of the global sample range through every stage.
+    // counters[k]: queue entering segment k; counters[kMaxSegments + k]: shadow rays emitted
+    // before segment k; [2 kMaxSegments, 4 kMaxSegments + 1): work counters of the launches.
+    template <int kMode> lmgpu_status batch(uint64_t base, uint32_t nb) {
+        hipStream_t s = ctx->stream;
+        c.base = base;
         c.batch = nb;
         hipLaunchKernelGGL(k_zero_u32, dim3(2), dim3(256), 0, s, b.counters, 512u);
         const dim3 grid_b((nb + kPTBlock - 1) / kPTBlock), grid_c((cap + kPTBlock - 1) / kPTBlock);
         // Persistent traversal grids: the device fill, never more than the queue capacity needs.
         const dim3 grid_t(std::min<uint32_t>(grid_c.x, trace_grid));
-        // counters[k]: queue entering segment k; counters[kMaxSegments + k]: shadow rays emitted before segment k.
-        e = timed(ctx, kKGen, [&]() {
-            hipLaunchKernelGGL(k_gen, grid_b, dim3(kPTBlock), 0, s, c, b.q[0], b.counters + 0, b.sq,
+        hipError_t e = timed(ctx, kKGen, [&]() {
+            hipLaunchKernelGGL(k_gen<kMode>, grid_b, dim3(kPTBlock), 0, s, c, b.q[0], b.counters + 0, b.sq,
                                b.counters + kMaxSegments + 0);
         });
         if (e != hipSuccess) return hip_fail(ctx, e, "k_gen");
-        if (!is_pt) {
-            e = timed(ctx, kKShadow, [&]() {
-                hipLaunchKernelGGL(k_shadow, grid_t, dim3(kTraceBlock), 0, s, c, b.sq, b.counters + kMaxSegments + 0,
-                                   b.counters + 3 * kMaxSegments + 0);
+        auto shadow = [&](int k) -> lmgpu_status {
+            hipError_t e2 = timed(ctx, kKShadow, [&]() {
+                hipLaunchKernelGGL(k_shadow<kMode>, grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
+                                   b.counters + kMaxSegments + k, b.counters + 3 * kMaxSegments + k);
             });
-            if (e != hipSuccess) return hip_fail(ctx, e, "k_shadow");
+            if (e2 != hipSuccess) return hip_fail(ctx, e2, "k_shadow");
+            if (kMode != kRender) {
+                hipLaunchKernelGGL(k_shadow_apply<kMode>, grid_c, dim3(kPTBlock), 0, s, c, b.sq,
+                                   b.counters + kMaxSegments + k);
+                e2 = hipGetLastError();
+                if (e2 != hipSuccess) return hip_fail(ctx, e2, "k_shadow_apply");
+            }
+            return LMGPU_OK;
+        };
+        if (!is_pt) {
+            lmgpu_status st = shadow(0);
+            if (st != LMGPU_OK) return st;
         }
         for (int k = 0; k < segments; k++) {
             const Queue& qi = b.q[k & 1];
@@ -854,64 +1043,204 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
             if (e != hipSuccess) return hip_fail(ctx, e, "k_extend");
             e = timed(ctx, kKShade, [&]() {
                 if (is_pt)
-                    hipLaunchKernelGGL(k_shade_pt, grid_c, dim3(kPTBlock), 0, s, c, qi, b.counters + k, b.hits, qo,
-                                       b.counters + k + 1);
+                    hipLaunchKernelGGL(k_shade_pt<kMode>, grid_c, dim3(kPTBlock), 0, s, c, qi, b.counters + k, b.hits,
+                                       qo, b.counters + k + 1);
                 else
-                    hipLaunchKernelGGL(k_shade_nee, grid_c, dim3(kPTBlock), 0, s, c, qi, b.counters + k, b.hits, qo,
-                                       b.counters + k + 1, b.sq, b.counters + kMaxSegments + k + 1);
+                    hipLaunchKernelGGL(k_shade_nee<kMode>, grid_c, dim3(kPTBlock), 0, s, c, qi, b.counters + k,
+                                       b.hits, qo, b.counters + k + 1, b.sq, b.counters + kMaxSegments + k + 1);
             });
             if (e != hipSuccess) return hip_fail(ctx, e, "k_shade");
             if (!is_pt) {
-                e = timed(ctx, kKShadow, [&]() {
-                    hipLaunchKernelGGL(k_shadow, grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
-                                       b.counters + kMaxSegments + k + 1, b.counters + 3 * kMaxSegments + k + 1);
-                });
-                if (e != hipSuccess) return hip_fail(ctx, e, "k_shadow");
+                lmgpu_status st = shadow(k + 1);
+                if (st != LMGPU_OK) return st;
             }
         }
-        // Paths still alive after the last segment end there; pt adds their radiance.
-        if (is_pt && segments >= 0) {
-            // k_shade_pt never enqueues a path whose vertex count reached max_path_vertices,
-            // so the final queue is empty; nothing to flush.
-        }
+        // k_shade_pt never enqueues a path whose vertex count reached max_path_vertices, so
+        // the final queue is empty.
         hipLaunchKernelGGL(k_accumulate_totals, dim3(1), dim3(64), 0, s, b.counters, segments + 1, b.totals);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(ctx, e, "k_accumulate_totals");
+        return LMGPU_OK;
     }
-    // Resolve: sums * W*H/N into the output film.
+};
+
+lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, lmgpu_pt_stats* stats) {
+    lmgpu_status st = validate(ctx, p, "lmgpu_pt_render");
+    if (st != LMGPU_OK) return st;
+    if (!film || !film->data || film->width != uint32_t(ctx->cam.width) || film->height != uint32_t(ctx->cam.height))
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: film size differs from the camera's");
+    if (film->precision != LMGPU_FILM_FP32 && film->precision != LMGPU_FILM_FP64)
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_render: output film precision");
+
+    Pipeline pl{ctx};
+    st = pl.setup(p);
+    if (st != LMGPU_OK) return st;
+    const size_t npix = size_t(ctx->cam.width) * size_t(ctx->cam.height);
     const size_t nval = npix * 3;
-    const dim3 rg(uint32_t((nval + 255) / 256));
-    if (p->film_precision == LMGPU_FILM_FP64) {
-        const double w = double(npix) / double(p->num_samples);
-        if (film->precision == LMGPU_FILM_FP64)
-            hipLaunchKernelGGL((k_resolve<double, double>), rg, dim3(256), 0, s, static_cast<const double*>(ctx->film_acc),
-                               static_cast<double*>(out_dev), nval, w);
-        else
-            hipLaunchKernelGGL((k_resolve<double, float>), rg, dim3(256), 0, s, static_cast<const double*>(ctx->film_acc),
-                               static_cast<float*>(out_dev), nval, w);
-    } else {
-        const float w = float(npix) / float(p->num_samples);
-        if (film->precision == LMGPU_FILM_FP64)
-            hipLaunchKernelGGL((k_resolve<float, double>), rg, dim3(256), 0, s, static_cast<const float*>(ctx->film_acc),
-                               static_cast<double*>(out_dev), nval, w);
-        else
-            hipLaunchKernelGGL((k_resolve<float, float>), rg, dim3(256), 0, s, static_cast<const float*>(ctx->film_acc),
-                               static_cast<float*>(out_dev), nval, w);
+    const bool comp = p->film_precision == LMGPU_FILM_FP32_COMPENSATED;
+    const size_t acc_elem = p->film_precision == LMGPU_FILM_FP64 ? 8 : 4;
+    const size_t acc_one = (nval * acc_elem + 15) / 16 * 16;
+    const size_t acc_bytes = comp ? 2 * acc_one : acc_one;
+    st = ensure(ctx, &ctx->film_acc, &ctx->film_acc_bytes, acc_bytes, "film accumulator");
+    if (st != LMGPU_OK) return st;
+    const size_t out_elem = film->precision == LMGPU_FILM_FP64 ? 8 : 4;
+    const size_t out_bytes = nval * out_elem;
+    void* out_dev = film->data;
+    if (!film->device) {
+        st = ensure(ctx, &ctx->film_out_staging, &ctx->film_out_staging_bytes, out_bytes, "film staging");
+        if (st != LMGPU_OK) return st;
+        out_dev = ctx->film_out_staging;
     }
-    e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(ctx, e, "k_resolve");
+    pl.c.film = ctx->film_acc;
+    pl.c.film_comp = comp ? reinterpret_cast<float*>(static_cast<char*>(ctx->film_acc) + acc_one) : nullptr;
+
+    hipStream_t s = ctx->stream;
+    hipEvent_t r0, r1;
+    (void)hipEventCreate(&r0);
+    (void)hipEventCreate(&r1);
+    (void)hipEventRecord(r0, s);
+    hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((acc_bytes / 16 + 255) / 256)), dim3(256), 0, s,
+                       static_cast<uint4*>(ctx->film_acc), acc_bytes / 16);
+    hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, pl.b.counters, (4096u + 64u) / 4u);
+    for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap)
+        st = pl.batch<kRender>(p->sample_offset + base, uint32_t(std::min<uint64_t>(pl.cap, p->num_samples - base)));
+    if (st == LMGPU_OK) {
+        // Resolve: sums * W*H/N into the output film.
+        const dim3 rg(uint32_t((nval + 255) / 256));
+        if (comp) {
+            const double w = double(npix) / double(p->num_samples);
+            const float* acc = static_cast<const float*>(ctx->film_acc);
+            if (film->precision == LMGPU_FILM_FP64)
+                hipLaunchKernelGGL(k_resolve_comp<double>, rg, dim3(256), 0, s, acc, pl.c.film_comp,
+                                   static_cast<double*>(out_dev), nval, w);
+            else
+                hipLaunchKernelGGL(k_resolve_comp<float>, rg, dim3(256), 0, s, acc, pl.c.film_comp,
+                                   static_cast<float*>(out_dev), nval, w);
+        } else if (p->film_precision == LMGPU_FILM_FP64) {
+            const double w = double(npix) / double(p->num_samples);
+            const double* acc = static_cast<const double*>(ctx->film_acc);
+            if (film->precision == LMGPU_FILM_FP64)
+                hipLaunchKernelGGL((k_resolve<double, double>), rg, dim3(256), 0, s, acc, static_cast<double*>(out_dev),
+                                   nval, w);
+            else
+                hipLaunchKernelGGL((k_resolve<double, float>), rg, dim3(256), 0, s, acc, static_cast<float*>(out_dev),
+                                   nval, w);
+        } else {
+            const float w = float(npix) / float(p->num_samples);
+            const float* acc = static_cast<const float*>(ctx->film_acc);
+            if (film->precision == LMGPU_FILM_FP64)
+                hipLaunchKernelGGL((k_resolve<float, double>), rg, dim3(256), 0, s, acc, static_cast<double*>(out_dev),
+                                   nval, w);
+            else
+                hipLaunchKernelGGL((k_resolve<float, float>), rg, dim3(256), 0, s, acc, static_cast<float*>(out_dev),
+                                   nval, w);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = hip_fail(ctx, e, "k_resolve");
+    }
     (void)hipEventRecord(r1, s);
     unsigned long long totals[2] = {0, 0};
-    e = hipMemcpyAsync(totals, b.totals, sizeof(totals), hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return hip_fail(ctx, e, "totals readback");
-    if (!film->device) {
-        e = hipMemcpyAsync(film->data, out_dev, out_bytes, hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) return hip_fail(ctx, e, "film readback");
+    if (st == LMGPU_OK) {
+        hipError_t e = hipMemcpyAsync(totals, pl.b.totals, sizeof(totals), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && !film->device) e = hipMemcpyAsync(film->data, out_dev, out_bytes, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(ctx, e, "pt readback");
     }
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(ctx, e, "pt synchronize");
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, r0, r1);
+    if (st == LMGPU_OK) (void)hipEventElapsedTime(&ms, r0, r1);
     (void)hipEventDestroy(r0);
     (void)hipEventDestroy(r1);
+    if (st != LMGPU_OK) return st;
+    if (stats) {
+        stats->closest_rays = totals[0];
+        stats->shadow_rays = totals[1];
+        stats->samples = p->num_samples;
+        stats->seconds = double(ms) / 1000.0;
+    }
+    return LMGPU_OK;
+}
+
+lmgpu_status run_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_adjoint* adj, lmgpu_pt_stats* stats) {
+    lmgpu_status st = validate(ctx, p, "lmgpu_pt_backward");
+    if (st != LMGPU_OK) return st;
+    if (!adj || !adj->d_image || !adj->d_albedo || (ctx->num_lights > 0 && !adj->d_emission))
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_pt_backward: missing image gradient or outputs");
+
+    Pipeline pl{ctx};
+    st = pl.setup(p);
+    if (st != LMGPU_OK) return st;
+    const size_t npix = size_t(ctx->cam.width) * size_t(ctx->cam.height);
+    const size_t nval = npix * 3;
+    const uint32_t ngrad = 3u * (ctx->num_materials + ctx->num_lights);
+    // Q per batch sample (4 doubles), the gradients, two residual sums, the staged dL/dI.
+    const size_t q_bytes = size_t(pl.cap) * 32;
+    const size_t g_bytes = (size_t(ngrad) * 8 + 15) / 16 * 16;
+    const size_t img_bytes = adj->image_device ? 0 : (nval * 4 + 15) / 16 * 16;
+    const size_t need = q_bytes + g_bytes + 16 + img_bytes;
+    st = ensure(ctx, &ctx->adj_buf, &ctx->adj_buf_bytes, need, "adjoint buffers");
+    if (st != LMGPU_OK) return st;
+    char* ab = static_cast<char*>(ctx->adj_buf);
+    pl.c.qacc = reinterpret_cast<double*>(ab);
+    double* grad = reinterpret_cast<double*>(ab + q_bytes);
+    double* sums = reinterpret_cast<double*>(ab + q_bytes + g_bytes);
+    pl.c.grad_mat = grad;
+    pl.c.grad_light = grad + 3ull * ctx->num_materials;
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    if (adj->image_device) {
+        pl.c.dLdI = adj->d_image;
+    } else {
+        float* img = reinterpret_cast<float*>(ab + q_bytes + g_bytes + 16);
+        e = hipMemcpyAsync(img, adj->d_image, nval * 4, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "image gradient upload");
+        pl.c.dLdI = img;
+    }
+    pl.c.film_mode = LMGPU_FILM_FP32;
+
+    hipEvent_t r0, r1;
+    (void)hipEventCreate(&r0);
+    (void)hipEventCreate(&r1);
+    (void)hipEventRecord(r0, s);
+    hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t(((g_bytes + 16) / 16 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint4*>(grad), (g_bytes + 16) / 16);
+    hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, pl.b.counters, (4096u + 64u) / 4u);
+    for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap) {
+        const uint32_t nb = uint32_t(std::min<uint64_t>(pl.cap, p->num_samples - base));
+        hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((size_t(nb) * 2 + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<uint4*>(pl.c.qacc), size_t(nb) * 2);
+        st = pl.batch<kAdjTotal>(p->sample_offset + base, nb);
+        if (st != LMGPU_OK) break;
+        hipLaunchKernelGGL(k_abs_sum, dim3((nb + 255) / 256), dim3(256), 0, s, pl.c.qacc, nb, sums + 0);
+        st = pl.batch<kAdjGrad>(p->sample_offset + base, nb);
+        if (st != LMGPU_OK) break;
+        hipLaunchKernelGGL(k_abs_sum, dim3((nb + 255) / 256), dim3(256), 0, s, pl.c.qacc, nb, sums + 1);
+    }
+    if (st == LMGPU_OK) {
+        // The film is Sum * W*H/N (sched.mt.cpp:256): the same factor scales its gradient.
+        hipLaunchKernelGGL(k_scale_f64, dim3((ngrad + 255) / 256), dim3(256), 0, s, grad, ngrad,
+                           double(npix) / double(p->num_samples));
+        e = hipGetLastError();
+        if (e != hipSuccess) st = hip_fail(ctx, e, "k_scale_f64");
+    }
+    (void)hipEventRecord(r1, s);
+    unsigned long long totals[2] = {0, 0};
+    double hs[2] = {0.0, 0.0};
+    if (st == LMGPU_OK) {
+        const hipMemcpyKind k = adj->grad_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        e = hipMemcpyAsync(adj->d_albedo, grad, size_t(ctx->num_materials) * 24, k, s);
+        if (e == hipSuccess && ctx->num_lights > 0)
+            e = hipMemcpyAsync(adj->d_emission, pl.c.grad_light, size_t(ctx->num_lights) * 24, k, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(totals, pl.b.totals, sizeof(totals), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(ctx, e, "adjoint readback");
+    }
+    float ms = 0.f;
+    if (st == LMGPU_OK) (void)hipEventElapsedTime(&ms, r0, r1);
+    (void)hipEventDestroy(r0);
+    (void)hipEventDestroy(r1);
+    if (st != LMGPU_OK) return st;
+    adj->replay_residual = hs[0] > 0.0 ? hs[1] / hs[0] : 0.0;
     if (stats) {
         stats->closest_rays = totals[0];
         stats->shadow_rays = totals[1];

@@ -13,7 +13,7 @@ from typing import Optional
 import numpy as np
 
 from . import _native as N
-from .scenes import FLT_MAX, Rays, Scene
+from .scenes import FLT_MAX, LIGHT_DTYPE, MATERIAL_DTYPE, Rays, Scene
 
 RENDERERS = {"pt": N.LMGPU_PT, "pt.direct": N.LMGPU_PT_DIRECT, "pt.mis": N.LMGPU_PT_MIS}
 FILM_PRECISIONS = {"fp32": N.LMGPU_FILM_FP32, "fp64": N.LMGPU_FILM_FP64,
@@ -102,20 +102,15 @@ class Device:
             M = np.ascontiguousarray(scene.material_ids, dtype=np.uint32)
             L = np.ascontiguousarray(scene.light_ids if scene.light_ids is not None
                                      else np.full(F.shape[0], -1), dtype=np.int32)
-            mats = (N.Material * len(scene.materials))()
-            for i, m in enumerate(scene.materials):
-                mats[i].type = int(m["type"])
-                mats[i].R[:] = [float(x) for x in m["R"]]
+            # MATERIAL_DTYPE / LIGHT_DTYPE have the layout of lmgpu_material / lmgpu_light.
+            mats = np.ascontiguousarray(scene.materials, dtype=MATERIAL_DTYPE)
             nl = 0 if scene.lights is None else len(scene.lights)
-            lights = (N.Light * max(nl, 1))()
-            for i in range(nl):
-                lights[i].type = int(scene.lights[i]["type"])
-                lights[i].Le[:] = [float(x) for x in scene.lights[i]["Le"]]
+            lights = np.ascontiguousarray(scene.lights if nl else np.zeros(1, LIGHT_DTYPE), dtype=LIGHT_DTYPE)
             d.material_ids = _ptr(M, C.c_uint32)
             d.light_ids = _ptr(L, C.c_int32)
-            d.materials = mats
-            d.num_materials = len(scene.materials)
-            d.lights = lights
+            d.materials = C.cast(mats.ctypes.data, C.POINTER(N.Material))
+            d.num_materials = len(mats)
+            d.lights = C.cast(lights.ctypes.data, C.POINTER(N.Light))
             d.num_lights = nl
             keep += [M, L, mats, lights]
         if scene.camera is not None:
@@ -229,6 +224,67 @@ class Device:
         self._check(self._lib.lmgpu_pt_render(self._ctx, C.byref(p), C.byref(film), C.byref(stats)))
         return out, RenderStats(int(stats.closest_rays), int(stats.shadow_rays), int(stats.samples),
                                 float(stats.seconds))
+
+    def set_params(self, albedo=None, emission=None) -> None:
+        """Replace material reflectances ((num_materials, 3)) and / or light emissions
+        ((num_lights, 3)) of the uploaded scene in place."""
+        a = None if albedo is None else np.ascontiguousarray(np.asarray(albedo, np.float32).reshape(-1, 3))
+        e = None if emission is None else np.ascontiguousarray(np.asarray(emission, np.float32).reshape(-1, 3))
+        sc = self.scene
+        if a is not None and (sc is None or a.shape[0] != len(sc.materials)):
+            raise LmgpuError(1, "albedo must have one row per material")
+        if e is not None and (sc is None or sc.lights is None or e.shape[0] != len(sc.lights)):
+            raise LmgpuError(1, "emission must have one row per light")
+        self._check(self._lib.lmgpu_scene_set_params(self._ctx, None if a is None else a.ctypes.data,
+                                                     None if e is None else e.ctypes.data))
+
+    def backward(self, renderer: str, d_image, num_samples: Optional[int] = None,
+                 max_path_vertices: Optional[int] = None, rr_depth: Optional[int] = None,
+                 seed: Optional[int] = None, sample_offset: int = 0, wavefront_width: int = 0,
+                 d_albedo_ptr: Optional[int] = None, d_emission_ptr: Optional[int] = None):
+        """Path-replay adjoint of render(renderer, ...) with the same arguments.
+
+        d_image: dL/dI, (H, W, 3) float32 numpy array, or an int device pointer to such data.
+        Returns (d_albedo (num_materials, 3), d_emission (num_lights, 3), info) -- the arrays
+        are None when device output pointers are given.
+        """
+        sc = self.scene
+        if sc is None or sc.camera is None:
+            raise LmgpuError(3, "upload a scene with a camera first")
+        rc = sc.render
+        p = N.PtParams()
+        p.renderer = RENDERERS[renderer]
+        p.num_samples = int(num_samples if num_samples is not None else rc.num_samples)
+        p.max_path_vertices = int(max_path_vertices if max_path_vertices is not None else rc.max_path_vertices)
+        p.rr_depth = int(rr_depth if rr_depth is not None else rc.rr_depth)
+        p.seed = int(seed if seed is not None else rc.seed)
+        p.sample_offset = int(sample_offset)
+        p.wavefront_width = int(wavefront_width)
+        p.film_precision = N.LMGPU_FILM_FP64
+        adj = N.Adjoint()
+        keep = []
+        if isinstance(d_image, int):
+            adj.d_image, adj.image_device = d_image, 1
+        else:
+            g = np.ascontiguousarray(d_image, dtype=np.float32)
+            if g.shape != (sc.camera.height, sc.camera.width, 3):
+                raise LmgpuError(1, "d_image must be (H, W, 3)")
+            keep.append(g)
+            adj.d_image, adj.image_device = g.ctypes.data, 0
+        nm = len(sc.materials)
+        nl = 0 if sc.lights is None else len(sc.lights)
+        ga = ge = None
+        if d_albedo_ptr is not None:
+            adj.d_albedo, adj.d_emission, adj.grad_device = int(d_albedo_ptr), d_emission_ptr, 1
+        else:
+            ga = np.zeros((nm, 3), np.float64)
+            ge = np.zeros((max(nl, 1), 3), np.float64)
+            adj.d_albedo, adj.d_emission, adj.grad_device = ga.ctypes.data, ge.ctypes.data, 0
+        stats = N.PtStats()
+        self._check(self._lib.lmgpu_pt_backward(self._ctx, C.byref(p), C.byref(adj), C.byref(stats)))
+        info = {"replay_residual": float(adj.replay_residual), "seconds": float(stats.seconds),
+                "closest_rays": int(stats.closest_rays), "shadow_rays": int(stats.shadow_rays)}
+        return ga, (None if ge is None else ge[:nl]), info
 
     # ---- timing / test hooks ---------------------------------------------------------
     def enable_timing(self, on: bool = True) -> None:

@@ -201,6 +201,95 @@ def room_scene(width: int = 256, height: int = 256, spp: int = 16, seed: int = 1
     return Scene("room", P, N, F, M, L, mats, lights, cam, rc)
 
 
+def _axis_boxes(centers: np.ndarray, half: np.ndarray, subdiv: int, top_only_grid: bool = False,
+                skip_bottom: bool = False):
+    """Axis-aligned boxes, outward normals, each face split into subdiv x subdiv quads
+    (with top_only_grid only the +y face is split; the sides are single quads).
+    Returns P (n*V, 3), N (n*V, 3), F (n*T, 3) and the number of triangles per box."""
+    n = centers.shape[0]
+    quads = []  # per quad: corner (n, 3), du (n, 3), dv (n, 3), normal (3,)
+    for k in range(3):
+        i, j = (k + 1) % 3, (k + 2) % 3
+        for sgn in (1.0, -1.0):
+            if skip_bottom and k == 1 and sgn < 0:
+                continue
+            a, b = (i, j) if sgn > 0 else (j, i)  # e_a x e_b = sgn e_k
+            s = subdiv if (not top_only_grid or (k == 1 and sgn > 0)) else 1
+            for ia in range(s):
+                for ib in range(s):
+                    corner = centers.copy()
+                    corner[:, k] += sgn * half[:, k]
+                    corner[:, a] += -half[:, a] + 2.0 * half[:, a] * ia / s
+                    corner[:, b] += -half[:, b] + 2.0 * half[:, b] * ib / s
+                    du = np.zeros_like(centers)
+                    dv = np.zeros_like(centers)
+                    du[:, a] = 2.0 * half[:, a] / s
+                    dv[:, b] = 2.0 * half[:, b] / s
+                    nrm = np.zeros(3)
+                    nrm[k] = sgn
+                    quads.append((corner, du, dv, nrm))
+    q = len(quads)
+    P = np.empty((n, q, 4, 3), dtype=np.float64)
+    Nn = np.empty((n, q, 4, 3), dtype=np.float64)
+    for t, (corner, du, dv, nrm) in enumerate(quads):
+        P[:, t, 0] = corner
+        P[:, t, 1] = corner + du
+        P[:, t, 2] = corner + du + dv
+        P[:, t, 3] = corner + dv
+        Nn[:, t, :] = nrm
+    base = (np.arange(n * q, dtype=np.int64) * 4)[:, None]
+    F = np.concatenate([base + np.array([0, 1, 2]), base + np.array([0, 2, 3])], axis=1).reshape(-1, 3)
+    return P.reshape(-1, 3), Nn.reshape(-1, 3), F, 2 * q
+
+
+def gradient_scene(width: int = 512, height: int = 512, spp: int = 32, num_boxes: int = 2000, seed: int = 0xC4,
+                   box_albedo=None, max_path_vertices: int = 9, rr_depth: int = 9) -> Scene:
+    """The room of room_scene plus `num_boxes` axis-aligned boxes (edges U[0.01, 0.05],
+    centres in [0.05, 0.95] x [0, 0.9] x [0.05, 0.95], each face a 2 x 2 grid: 48 triangles)
+    with one diffuse material per box, and four 0.2 x 0.2 ceiling lights.  Materials:
+    0 white, 1 red, 2 green, 3 black (lights), 4 + i box i.  rr_depth = max_path_vertices
+    disables Russian roulette, so with common sample streams the image is a polynomial in
+    the reflectances and emissions."""
+    b = _Builder()
+    WHITE, RED, GREEN, BLACK = 0, 1, 2, 3
+    b.quad((0, 0, 0), (0, 0, 1), (1, 0, 0), WHITE)
+    b.quad((0, 1, 0), (1, 0, 0), (0, 0, 1), WHITE)
+    b.quad((0, 0, 0), (1, 0, 0), (0, 1, 0), WHITE)
+    b.quad((0, 0, 0), (0, 1, 0), (0, 0, 1), RED)
+    b.quad((1, 0, 0), (0, 0, 1), (0, 1, 0), GREEN)
+    b.box((0.35, 0.15, 0.35), (0.3, 0.3, 0.3), 18.0, WHITE)
+    b.box((0.68, 0.30, 0.62), (0.3, 0.6, 0.3), -20.0, WHITE)
+    for li, (cx, cz) in enumerate(((0.3, 0.3), (0.7, 0.3), (0.3, 0.7), (0.7, 0.7))):
+        b.quad((cx - 0.1, 0.999, cz - 0.1), (0.2, 0, 0), (0, 0, 0.2), BLACK, light=li)
+    P0, N0, F0, M0, L0 = b.arrays()
+    u = uniform_blocks(seed, num_boxes, 2, TAG_GEOMETRY)
+    half = (0.01 + 0.04 * u[:, 0:3]) / 2.0
+    ctr = np.stack([0.05 + 0.9 * u[:, 3], 0.9 * u[:, 4], 0.05 + 0.9 * u[:, 5]], axis=1)
+    P1, N1, F1, tpb = _axis_boxes(ctr, half, 2)
+    P = np.concatenate([P0, P1.astype(np.float32)])
+    Nn = np.concatenate([N0, N1.astype(np.float32)])
+    F = np.concatenate([F0, (F1 + P0.shape[0]).astype(np.uint32)])
+    M = np.concatenate([M0, np.repeat(4 + np.arange(num_boxes, dtype=np.uint32), tpb)])
+    L = np.concatenate([L0, np.full(num_boxes * tpb, -1, dtype=np.int32)])
+    if box_albedo is None:
+        box_albedo = np.full((num_boxes, 3), 0.5)
+    entries = [(BSDF_DIFFUSE, (0.75, 0.75, 0.75)), (BSDF_DIFFUSE, (0.63, 0.06, 0.04)),
+               (BSDF_DIFFUSE, (0.14, 0.45, 0.09)), (BSDF_DIFFUSE, (0.0, 0.0, 0.0))]
+    mats = _materials(entries + [(BSDF_DIFFUSE, tuple(r)) for r in np.asarray(box_albedo, dtype=np.float64)])
+    lights = _lights([(LIGHT_AREA, (8.5, 6.0, 2.0))] * 4)
+    view = look_at((0.5, 0.5, 1.95), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0)).astype(np.float32)
+    cam = Camera(view=view, fovy=60.0, width=width, height=height)
+    rc = RenderConfig(num_samples=width * height * spp, max_path_vertices=max_path_vertices, rr_depth=rr_depth,
+                      seed=1)
+    return Scene("gradient", P, Nn, F, M, L, mats, lights, cam, rc,
+                 meta={"box_materials": 4 + np.arange(num_boxes), "seed": seed})
+
+
+def hidden_box_albedo(num_boxes: int = 2000, seed: int = 0xC4) -> np.ndarray:
+    """The hidden per-box reflectances of the gradient workload's target image: U[0.1, 0.9]^3."""
+    return 0.1 + 0.8 * uniform_blocks(seed ^ 0x7A46, num_boxes, 1, TAG_GEOMETRY)[:, 0:3]
+
+
 def traversal_scene(num_triangles: int = 1_000_000, seed: int = 0xC2A, sigma: float = 0.01) -> Scene:
     """Triangles with centroids uniform in [0,1]^3 and vertices at centroid + N(0, sigma^2 I)."""
     u = uniform_blocks(seed, num_triangles, 4, TAG_GEOMETRY)  # 16 uniforms per triangle

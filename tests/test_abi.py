@@ -30,9 +30,14 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
+    """The library reports the LMGPU_ABI_VERSION its header declares."""
+    import re
+
     from lightmetrica_amd import _native
 
-    assert _native.load().lmgpu_abi_version() == 1
+    header = (Path(__file__).resolve().parents[1] / "include" / "lmgpu.h").read_text()
+    declared = int(re.search(r"#define LMGPU_ABI_VERSION (\d+)", header).group(1))
+    assert _native.load().lmgpu_abi_version() == declared
 
 
 def test_null_context_is_rejected_without_gpu():

@@ -76,6 +76,23 @@ def test_fp32_film_matches_fp64_film(gpu):
     np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6)
 
 
+def test_compensated_fp32_film(gpu):
+    """Many samples per pixel (a 4 x 4 film, 2^22 samples: 2^18 per pixel): the FP32 sums
+    drift from the FP64 sums of the same contributions, the compensated FP32 sums do not."""
+    sc = scenes.room_scene(4, 4, 1)
+    gpu.upload(sc)
+    gpu.build_bvh()
+    n = 1 << 22
+    ref, _ = gpu.render("pt.direct", num_samples=n, film_precision="fp64")
+    f32, _ = gpu.render("pt.direct", num_samples=n, film_precision="fp32")
+    comp, _ = gpu.render("pt.direct", num_samples=n, film_precision="fp32c")
+    e32 = np.abs(f32 - ref).max() / np.abs(ref).max()
+    ec = np.abs(comp - ref).max() / np.abs(ref).max()
+    print(f"max relative film error vs FP64 sums: FP32 {e32:.2e}, compensated FP32 {ec:.2e}", flush=True)
+    assert ec < 1e-9
+    assert ec < e32
+
+
 def test_wavefront_width_does_not_change_the_image(gpu):
     """Batches are split by global sample index, so the batch size only changes summation order."""
     sc = scenes.room_scene(32, 32, 16)
