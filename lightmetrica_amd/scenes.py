@@ -290,6 +290,132 @@ def hidden_box_albedo(num_boxes: int = 2000, seed: int = 0xC4) -> np.ndarray:
     return 0.1 + 0.8 * uniform_blocks(seed ^ 0x7A46, num_boxes, 1, TAG_GEOMETRY)[:, 0:3]
 
 
+def _gradient_noise(x: np.ndarray, z: np.ndarray, freq: int, seed: int) -> np.ndarray:
+    """2-D gradient noise (Perlin, quintic fade) on a freq x freq lattice over [0, 1]^2,
+    unit gradients at angles drawn from Philox4x32-10."""
+    ang = 2.0 * np.pi * uniform_blocks(seed, (freq + 1) * (freq + 1), 1, TAG_GEOMETRY)[:, 0]
+    gx, gz = np.cos(ang).reshape(freq + 1, freq + 1), np.sin(ang).reshape(freq + 1, freq + 1)
+    fx, fz = x * freq, z * freq
+    ix = np.minimum(np.floor(fx).astype(np.int64), freq - 1)
+    iz = np.minimum(np.floor(fz).astype(np.int64), freq - 1)
+    tx, tz = fx - ix, fz - iz
+
+    def corner(dx, dz):
+        return gx[ix + dx, iz + dz] * (tx - dx) + gz[ix + dx, iz + dz] * (tz - dz)
+
+    fade = lambda t: t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+    u, v = fade(tx), fade(tz)
+    a = corner(0, 0) + u * (corner(1, 0) - corner(0, 0))
+    b = corner(0, 1) + u * (corner(1, 1) - corner(0, 1))
+    return a + v * (b - a)
+
+
+def terrain_scene(width: int = 1920, height: int = 1080, spp: int = 64, grid: int = 1024, seed: int = 0xC3,
+                  mirror_fraction: float = 0.4) -> Scene:
+    """Heightfield terrain: (grid + 1)^2 vertices over [0, 1]^2 (x, z), height 0.2 * fBm
+    (6 octaves of gradient noise from 4 cells, lacunarity 2, gain 0.5), 2 grid^2 triangles
+    with interpolated normals.  Per triangle: a mirror (R = 1) with probability 0.4, else a
+    diffuse material of its own with reflectance U[0.1, 0.9]^3.  A 1 x 1 quad light at
+    y = 1.5 facing down (Le = 10) and a constant environment (Le = 0.5, first light)."""
+    n = grid + 1
+    c = np.arange(n, dtype=np.float64) / grid
+    X, Z = np.meshgrid(c, c, indexing="ij")
+    Y = np.zeros_like(X)
+    amp, freq = 1.0, 4
+    for o in range(6):
+        Y += amp * _gradient_noise(X, Z, freq, seed + 101 * o)
+        amp *= 0.5
+        freq *= 2
+    Y *= 0.2
+    # Normals from central differences of the height field.
+    dx = np.gradient(Y, c, axis=0)
+    dz = np.gradient(Y, c, axis=1)
+    Nv = np.stack([-dx, np.ones_like(Y), -dz], axis=-1)
+    Nv /= np.linalg.norm(Nv, axis=-1, keepdims=True)
+    P = np.stack([X, Y, Z], axis=-1).reshape(-1, 3)
+    idx = np.arange(n * n, dtype=np.int64).reshape(n, n)
+    v00, v10 = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel()
+    v01, v11 = idx[:-1, 1:].ravel(), idx[1:, 1:].ravel()
+    F = np.empty((2 * grid * grid, 3), dtype=np.int64)
+    F[0::2] = np.stack([v00, v01, v11], axis=1)
+    F[1::2] = np.stack([v00, v11, v10], axis=1)
+    nt = F.shape[0]
+    u = uniform_blocks(seed ^ 0x5EED, nt, 1, TAG_GEOMETRY)
+    mirror = u[:, 0] < mirror_fraction
+    # Material 0: black (light), 1: the shared mirror, 2 + k: the k-th diffuse triangle.
+    M = np.empty(nt, dtype=np.uint32)
+    M[mirror] = 1
+    nd = int((~mirror).sum())
+    M[~mirror] = 2 + np.arange(nd, dtype=np.uint32)
+    mats = np.zeros(2 + nd, dtype=MATERIAL_DTYPE)
+    mats[0]["type"], mats[1]["type"] = BSDF_DIFFUSE, BSDF_MIRROR
+    mats[1]["R"] = (1.0, 1.0, 1.0)
+    mats["type"][2:] = BSDF_DIFFUSE
+    mats["R"][2:] = 0.1 + 0.8 * u[~mirror, 1:4]
+    # The light quad: corner (0, 1.5, 0), spanning x and z, normal -y.
+    lp = np.array([[0, 1.5, 0], [1, 1.5, 0], [1, 1.5, 1], [0, 1.5, 1]], dtype=np.float64)
+    base = P.shape[0]
+    P = np.concatenate([P, lp]).astype(np.float32)
+    Nn = np.concatenate([Nv.reshape(-1, 3), np.tile([0.0, -1.0, 0.0], (4, 1))]).astype(np.float32)
+    F = np.concatenate([F, [[base, base + 1, base + 2], [base, base + 2, base + 3]]]).astype(np.uint32)
+    M = np.concatenate([M, [0, 0]]).astype(np.uint32)
+    L = np.concatenate([np.full(nt, -1), [1, 1]]).astype(np.int32)
+    lights = _lights([(LIGHT_ENV_CONST, (0.5, 0.5, 0.5)), (LIGHT_AREA, (10.0, 10.0, 10.0))])
+    view = look_at((0.5, 0.6, 1.6), (0.5, 0.1, 0.5), (0.0, 1.0, 0.0)).astype(np.float32)
+    cam = Camera(view=view, fovy=45.0, width=width, height=height)
+    rc = RenderConfig(num_samples=width * height * spp, max_path_vertices=17, rr_depth=3, seed=1)
+    return Scene("terrain", P, Nn, F, M, L, mats, lights, cam, rc, meta={"seed": seed, "grid": grid})
+
+
+def city_scene(width: int = 3840, height: int = 2160, spp: int = 256, cells: int = 1000, seed: int = 0xC5,
+               groups: int = 10_000, num_lights: int = 16) -> Scene:
+    """Procedural city: one axis-aligned box per unit cell of a cells x cells grid, footprint
+    0.8 x 0.8, height exp(N(0, 0.8^2)); 16 triangles per box (4 sides, a 2 x 2 top, no
+    bottom).  Box i takes albedo group (2654435761 i mod 2^32) mod `groups`, each group a
+    diffuse reflectance U[0.1, 0.9]^3.  A ground quad (reflectance 0.5) and a 4 x 4 grid of
+    40 x 40 quad lights at y = 60 facing down.  Materials: 0 .. groups-1 the groups, then
+    the ground, then black (lights)."""
+    n = cells * cells
+    u = uniform_blocks(seed, n, 1, TAG_GEOMETRY)
+    r = np.sqrt(-2.0 * np.log(1.0 - u[:, 0]))
+    hgt = np.exp(0.8 * r * np.cos(2.0 * np.pi * u[:, 1]))
+    ii, jj = np.meshgrid(np.arange(cells, dtype=np.float64), np.arange(cells, dtype=np.float64), indexing="ij")
+    ctr = np.stack([ii.ravel() + 0.5, hgt / 2.0, jj.ravel() + 0.5], axis=1)
+    half = np.stack([np.full(n, 0.4), hgt / 2.0, np.full(n, 0.4)], axis=1)
+    P, Nn, F, tpb = _axis_boxes(ctr, half, 2, top_only_grid=True, skip_bottom=True)
+    grp = ((np.arange(n, dtype=np.uint64) * np.uint64(2654435761)) & _MASK32) % np.uint64(groups)
+    M = np.repeat(grp.astype(np.uint32), tpb)
+    ground, black = groups, groups + 1
+    ext = float(cells)
+    gq = np.array([[0, 0, 0], [0, 0, ext], [ext, 0, ext], [ext, 0, 0]], dtype=np.float64)  # normal +y
+    lq = []
+    k = int(round(np.sqrt(num_lights)))
+    for a in range(k):
+        for b in range(k):
+            cx, cz = (a + 0.5) * ext / k, (b + 0.5) * ext / k
+            lq.append([[cx - 20, 60, cz - 20], [cx + 20, 60, cz - 20], [cx + 20, 60, cz + 20], [cx - 20, 60, cz + 20]])
+    extra = np.concatenate([gq[None], np.asarray(lq, dtype=np.float64)]).reshape(-1, 3)
+    base = P.shape[0]
+    quads = (base + 4 * np.arange(1 + len(lq)))[:, None]
+    Fx = np.concatenate([quads + np.array([0, 1, 2]), quads + np.array([0, 2, 3])], axis=1).reshape(-1, 3)
+    Nx = np.concatenate([np.tile([0.0, 1.0, 0.0], (4, 1)), np.tile([0.0, -1.0, 0.0], (4 * len(lq), 1))])
+    P = np.concatenate([P, extra]).astype(np.float32)
+    Nn = np.concatenate([Nn, Nx]).astype(np.float32)
+    F = np.concatenate([F, Fx]).astype(np.uint32)
+    M = np.concatenate([M, [ground, ground], np.full(2 * len(lq), black)]).astype(np.uint32)
+    L = np.concatenate([np.full(n * tpb + 2, -1), np.repeat(np.arange(len(lq)), 2)]).astype(np.int32)
+    alb = 0.1 + 0.8 * uniform_blocks(seed ^ 0xA1BE, groups, 1, TAG_GEOMETRY)[:, 0:3]
+    mats = np.zeros(groups + 2, dtype=MATERIAL_DTYPE)
+    mats["type"] = BSDF_DIFFUSE
+    mats["R"][:groups] = alb
+    mats["R"][ground] = (0.5, 0.5, 0.5)
+    lights = _lights([(LIGHT_AREA, (5.0, 5.0, 5.0))] * len(lq))
+    view = look_at((ext / 2, 120.0, ext * 1.08), (ext / 2, 0.0, ext * 0.45), (0.0, 1.0, 0.0)).astype(np.float32)
+    cam = Camera(view=view, fovy=40.0, width=width, height=height)
+    rc = RenderConfig(num_samples=width * height * spp, max_path_vertices=13, rr_depth=3, seed=1)
+    return Scene("city", P, Nn, F, M, L, mats, lights, cam, rc, meta={"seed": seed, "cells": cells})
+
+
 def traversal_scene(num_triangles: int = 1_000_000, seed: int = 0xC2A, sigma: float = 0.01) -> Scene:
     """Triangles with centroids uniform in [0,1]^3 and vertices at centroid + N(0, sigma^2 I)."""
     u = uniform_blocks(seed, num_triangles, 4, TAG_GEOMETRY)  # 16 uniforms per triangle

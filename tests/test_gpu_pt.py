@@ -78,7 +78,10 @@ def test_fp32_film_matches_fp64_film(gpu):
 
 def test_compensated_fp32_film(gpu):
     """Many samples per pixel (a 4 x 4 film, 2^22 samples: 2^18 per pixel): the FP32 sums
-    drift from the FP64 sums of the same contributions, the compensated FP32 sums do not."""
+    drift from the FP64 sums of the same contributions (small terms are rounded away once the
+    sum is large), the compensated FP32 sums stay close.  What remains is the FP32 rounding
+    of the error sums themselves, ~sqrt(n) eps times the FP32 drift (measured on MI355X:
+    FP32 6.5e-4, compensated 1.9e-8)."""
     sc = scenes.room_scene(4, 4, 1)
     gpu.upload(sc)
     gpu.build_bvh()
@@ -89,8 +92,8 @@ def test_compensated_fp32_film(gpu):
     e32 = np.abs(f32 - ref).max() / np.abs(ref).max()
     ec = np.abs(comp - ref).max() / np.abs(ref).max()
     print(f"max relative film error vs FP64 sums: FP32 {e32:.2e}, compensated FP32 {ec:.2e}", flush=True)
-    assert ec < 1e-9
-    assert ec < e32
+    assert ec < 1e-6
+    assert ec < e32 / 100
 
 
 def test_wavefront_width_does_not_change_the_image(gpu):
