@@ -201,6 +201,120 @@ def run_adjoint(args, Device, scenes):
     }
 
 
+def rmse(a, b):
+    """BitmapImage::EvaluateRMSE (bitmap.cpp:31-52): sqrt(mean over W*H*3 values of (a - b)^2)."""
+    import numpy as np
+
+    return float(np.sqrt(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)))
+
+
+def run_terrain_study(args, Device, scenes):
+    """Film precision study on the terrain: the same 132,710,400 path samples accumulated in
+    FP32, compensated FP32 and FP64 films, each compared with the FP64 film of the same
+    contributions and with the FP64 CPU restatement tracing the same sample streams."""
+    import numpy as np
+
+    from oracle import lmref
+
+    t0 = time.time()
+    spp = args.spp or 64
+    sc = scenes.terrain_scene(1920, 1080, spp)
+    log(f"[terrain] {sc.num_triangles} triangles, {len(sc.materials)} materials, generated in {time.time() - t0:.1f}s")
+    dev = Device(0)
+    dev.upload(sc)
+    info = dev.build_bvh()
+    log(f"[terrain] BVH {info.num_nodes} nodes, depth {info.max_depth}, {info.build_seconds:.1f}s")
+    dev.render("pt", num_samples=1 << 22)  # warm-up
+    films, out = {}, {}
+    for mode in ("fp32", "fp32c", "fp64"):
+        img, st = dev.render("pt", film_precision=mode, out_precision="fp64")
+        films[mode] = img
+        out[mode] = {"seconds": round(st.seconds, 4), "M_path_samples_per_s": round(st.samples / st.seconds / 1e6, 1),
+                     "Mrays_per_s": round((st.closest_rays + st.shadow_rays) / st.seconds / 1e6, 1)}
+        log(f"[terrain] {mode}: {st.seconds:.3f}s")
+    ref = films["fp64"]
+    scale = float(np.sqrt(np.mean(ref ** 2)))
+    for mode in ("fp32", "fp32c"):
+        out[mode]["rmse_vs_gpu_fp64_film"] = rmse(films[mode], ref)
+        out[mode]["max_rel_vs_gpu_fp64_film"] = float(np.max(np.abs(films[mode] - ref)) / np.max(np.abs(ref)))
+    # The FP64 oracle traces the same streams; compared over the whole image.
+    threads = host_threads()
+    t0 = time.time()
+    o = lmref.Oracle(sc, fp64=True, accel="bvh")
+    log(f"[terrain] oracle BVH built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    oimg, ost = o.render("pt", threads=threads)
+    log(f"[terrain] oracle rendered {ost['samples']} samples in {time.time() - t0:.1f}s on {threads} threads")
+    for mode in ("fp32", "fp32c", "fp64"):
+        d = films[mode] - oimg
+        out[mode]["rmse_vs_fp64_oracle"] = rmse(films[mode], oimg)
+        out[mode]["rel_rmse_vs_fp64_oracle"] = rmse(films[mode], oimg) / scale
+        out[mode]["pixels_within_1e-3"] = float(np.mean(np.abs(d) <= 1e-3 * np.maximum(np.abs(oimg), 1e-2 * scale)))
+    dev.close()
+    return {"workload": "terrain: 1024x1024-cell heightfield (2,097,152 triangles), 6-octave fBm x 0.2, 40% mirror "
+                        "triangles, 60% diffuse U[0.1,0.9]; 1x1 quad light (Le 10) + constant environment (0.5); "
+                        f"1920x1080, {spp} spp ({sc.render.num_samples} samples), max_path_vertices 17, rr_depth 3; "
+                        "renderer pt",
+            "image_rms": scale, "films": out,
+            "oracle": {"seconds": round(ost["seconds"], 2), "threads": threads, "cpu": cpu_info(),
+                       "M_path_samples_per_s": round(ost["samples"] / ost["seconds"] / 1e6, 3)}}
+
+
+def run_city(args, torch, Device, scenes):
+    """The 16M-triangle city: FP32 forward, path-replay backward (FP64 gradient reduction), HBM use."""
+    import numpy as np
+
+    t0 = time.time()
+    spp = args.spp or 256
+    sc = scenes.city_scene(3840, 2160, spp)
+    log(f"[city] {sc.num_triangles} triangles generated in {time.time() - t0:.1f}s")
+    free0, total = torch.cuda.mem_get_info(0)
+    dev = Device(0)
+    dev.upload(sc)
+    info = dev.build_bvh()
+    log(f"[city] BVH {info.num_nodes} nodes, depth {info.max_depth}, {info.build_seconds:.1f}s, "
+        f"{info.device_bytes / 2**30:.2f} GiB")
+    dev.render("pt", num_samples=1 << 22)  # warm-up
+    img, st = dev.render("pt", film_precision="fp32", out_precision="fp32")
+    log(f"[city] forward {st.samples} samples in {st.seconds:.2f}s")
+    free1, _ = torch.cuda.mem_get_info(0)
+    # Backward of L = 0.5 * sum(I^2): dL/dI = I.
+    ga, ge, binfo = dev.backward("pt", img.astype(np.float32))
+    log(f"[city] backward in {binfo['seconds']:.2f}s, replay residual {binfo['replay_residual']:.1e}")
+    free2, _ = torch.cuda.mem_get_info(0)
+    dev.close()
+    res = {"workload": f"city: 1000x1000 boxes (16,000,034 triangles), log-normal heights, 10k albedo groups, "
+                       f"16 quad lights; 3840x2160, {spp} spp ({sc.render.num_samples} samples), max_path_vertices 13, "
+                       f"rr_depth 3; renderer pt",
+           "forward_s": round(st.seconds, 3),
+           "M_path_samples_per_s": round(st.samples / st.seconds / 1e6, 1),
+           "Mrays_per_s": round((st.closest_rays + st.shadow_rays) / st.seconds / 1e6, 1),
+           "rays_per_sample": round((st.closest_rays + st.shadow_rays) / st.samples, 3),
+           "backward_s": round(binfo["seconds"], 3),
+           "backward_over_forward": round(binfo["seconds"] / st.seconds, 3),
+           "replay_residual": binfo["replay_residual"],
+           "bvh_build_s": round(info.build_seconds, 2),
+           "bvh_device_GiB": round(info.device_bytes / 2**30, 3),
+           "hbm_used_forward_GiB": round((free0 - free1) / 2**30, 3),
+           "hbm_used_peak_GiB": round((free0 - min(free1, free2)) / 2**30, 3),
+           "hbm_total_GiB": round(total / 2**30, 1)}
+    if not args.no_cpu_baseline:
+        from oracle import lmref
+
+        threads = host_threads()
+        n = sc.render.num_samples // 64
+        t0 = time.time()
+        o = lmref.Oracle(sc, fp64=False, accel="qbvh", approx_rsqrt=True)
+        log(f"[city] CPU QBVH built in {time.time() - t0:.1f}s")
+        _, cst = o.render("pt", num_samples=n, rng="mt", threads=threads)
+        res["cpu_baseline"] = {"value": cst["samples"] / cst["seconds"] / 1e6, "unit": "M path-samples/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"1/64 of the samples ({n}), FP32 SSE QBVH, mt scheduler, {threads} threads "
+                                         f"of {cpu_info()}; the full render extrapolates linearly to "
+                                         f"{sc.render.num_samples / (cst['samples'] / cst['seconds']):.0f} s"}
+    return res
+
+
 def cpu_baseline_traversal(sc, rays, sample_rays):
     from oracle import lmref
 
@@ -241,6 +355,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-sample-rays", type=int, default=16_000_000)
     ap.add_argument("--cpu-sample-paths", type=int, default=32 * 1_048_576)
+    ap.add_argument("--workload", choices=["traversal", "terrain", "city"], default="traversal",
+                    help="traversal: the headline line; terrain: the film-precision study; city: the 16M-triangle "
+                         "forward + backward scale run")
+    ap.add_argument("--spp", type=int, default=0, help="terrain / city: samples per pixel (0 = the workload's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -256,8 +374,9 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the path shards by sample index; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path shards by global "
+              f"sample index with one film reduction, and only the one-GPU share is measured here.",
+              file=sys.stderr)
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
@@ -269,6 +388,12 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
+    if args.workload == "terrain":
+        print(json.dumps(run_terrain_study(args, Device, scenes)), flush=True)
+        return 0
+    if args.workload == "city":
+        print(json.dumps(run_city(args, torch, Device, scenes)), flush=True)
+        return 0
     res, sc, rays = run_traversal(args, torch, Device, scenes)
     achieved = res["alg_bytes"] / (res["kernel_ms"] * 1e-3) / 1e9
     traffic = traffic_from_profile()

@@ -189,53 +189,72 @@ __device__ __forceinline__ void contribute(const PTConsts& c, uint32_t sid, uint
     }
 }
 
-// g[3 id + k] += (x, y, z)[k] for the active lanes.  Lanes holding the same id as the
-// first active lane are summed in registers first (a few rounds catch the walls and
-// lights most lanes share), the rest add on their own.  Every lane of the wave must call
-// it together.
-constexpr int kAggRounds = 3;
-__device__ __forceinline__ void scatter_grad(double* g, uint32_t id, double x, double y, double z, bool active) {
-#pragma unroll 1
-    for (int round = 0; round < kAggRounds; round++) {
-        const unsigned long long m = __ballot(active);
-        if (m == 0ull) return;
-        const int leader = __ffsll((long long)m) - 1;
-        const uint32_t lid = __shfl(id, leader, 64);
-        const bool mine = active && id == lid;
-        double sx = mine ? x : 0.0, sy = mine ? y : 0.0, sz = mine ? z : 0.0;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            sx += __shfl_xor(sx, off, 64);
-            sy += __shfl_xor(sy, off, 64);
-            sz += __shfl_xor(sz, off, 64);
-        }
-        if (int(__lane_id()) == leader) {
-            double* p = g + 3ull * lid;
-            if (sx != 0.0) unsafeAtomicAdd(p + 0, sx);
-            if (sy != 0.0) unsafeAtomicAdd(p + 1, sy);
-            if (sz != 0.0) unsafeAtomicAdd(p + 2, sz);
-        }
-        active = active && !mine;
+// Gradient accumulation of one block: an open-addressing hash table in LDS keyed by
+// material id, or by kLightKey | light id, holding FP64 sums.  Paths of a block share
+// few hot parameters (walls, lights), so the table turns most device-scope FP64 atomics
+// into LDS atomics; each block adds its table to the gradient once, at its end.  A key
+// that finds no slot within kGradProbes goes straight to the device atomics.
+constexpr int kGradSlots = 512;
+constexpr int kGradProbes = 8;
+constexpr uint32_t kEmptyKey = 0xffffffffu;
+constexpr uint32_t kLightKey = 0x80000000u;
+
+struct GradCache {
+    uint32_t keys[kGradSlots];
+    double vals[3 * kGradSlots];
+};
+
+__device__ __forceinline__ double* grad_target(const PTConsts& c, uint32_t key) {
+    return (key & kLightKey) ? c.grad_light + 3ull * (key & ~kLightKey) : c.grad_mat + 3ull * key;
+}
+
+__device__ __forceinline__ void gc_init(GradCache& g) {
+    for (int i = threadIdx.x; i < kGradSlots; i += blockDim.x) {
+        g.keys[i] = kEmptyKey;
+        g.vals[3 * i] = g.vals[3 * i + 1] = g.vals[3 * i + 2] = 0.0;
     }
-    if (active) {
-        double* p = g + 3ull * id;
-        if (x != 0.0) unsafeAtomicAdd(p + 0, x);
-        if (y != 0.0) unsafeAtomicAdd(p + 1, y);
-        if (z != 0.0) unsafeAtomicAdd(p + 2, z);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void gc_add(GradCache& g, const PTConsts& c, uint32_t key, double x, double y, double z) {
+    if (x == 0.0 && y == 0.0 && z == 0.0) return;
+    uint32_t slot = (key * 2654435761u) >> 23;  // 9 bits: kGradSlots = 512
+    for (int probe = 0; probe < kGradProbes; probe++) {
+        uint32_t k = __hip_atomic_load(&g.keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k == kEmptyKey) {
+            const uint32_t prev = atomicCAS(&g.keys[slot], kEmptyKey, key);
+            k = prev == kEmptyKey ? key : prev;
+        }
+        if (k == key) {
+            atomicAdd(&g.vals[3 * slot + 0], x);
+            atomicAdd(&g.vals[3 * slot + 1], y);
+            atomicAdd(&g.vals[3 * slot + 2], z);
+            return;
+        }
+        slot = (slot + 1) & (kGradSlots - 1);
+    }
+    double* d = grad_target(c, key);
+    if (x != 0.0) unsafeAtomicAdd(d + 0, x);
+    if (y != 0.0) unsafeAtomicAdd(d + 1, y);
+    if (z != 0.0) unsafeAtomicAdd(d + 2, z);
+}
+
+__device__ __forceinline__ void gc_flush(GradCache& g, const PTConsts& c) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kGradSlots; i += blockDim.x) {
+        const uint32_t key = g.keys[i];
+        if (key == kEmptyKey) continue;
+        double* d = grad_target(c, key);
+        for (int k = 0; k < 3; k++)
+            if (g.vals[3 * i + k] != 0.0) unsafeAtomicAdd(d + k, g.vals[3 * i + k]);
     }
 }
 
-// dL/dLe of a contribution v = w * Le (w: the contribution without Le), scattered to
-// light `li`.  Wave-uniform call.
-__device__ __forceinline__ void grad_emission(const PTConsts& c, uint32_t pixel, V3 w, int li, bool active) {
-    double x = 0.0, y = 0.0, z = 0.0;
-    if (active) {
-        const float* g = c.dLdI + 3ull * pixel;
-        x = double(g[0]) * double(w.x);
-        y = double(g[1]) * double(w.y);
-        z = double(g[2]) * double(w.z);
-    }
-    scatter_grad(c.grad_light, uint32_t(active ? li : 0), x, y, z, active);
+// dL/dLe of a contribution v = w * Le (w: the contribution without Le) of light `li`.
+__device__ __forceinline__ void grad_emission(GradCache& g, const PTConsts& c, uint32_t pixel, V3 w, int li) {
+    const float* dl = c.dLdI + 3ull * pixel;
+    gc_add(g, c, kLightKey | uint32_t(li), double(dl[0]) * double(w.x), double(dl[1]) * double(w.y),
+           double(dl[2]) * double(w.z));
 }
 
 // HDRBitmapFilm pixel of a raster position (hdrfilm.cpp:160-166); y = 0 is the bottom row.
@@ -625,25 +644,27 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict_
 }
 
 // dL/dR of the vertex's material: Q / R per channel (the contributions still in Q all carry
-// R once).  Wave-uniform call.
-__device__ __forceinline__ void grad_material(const PTConsts& c, uint32_t sid, uint32_t mat, V3 R, bool active) {
-    double x = 0.0, y = 0.0, z = 0.0;
-    if (active) {
-        const double* q = c.qacc + 4ull * sid;
-        x = R.x != 0.f ? q[0] / double(R.x) : 0.0;
-        y = R.y != 0.f ? q[1] / double(R.y) : 0.0;
-        z = R.z != 0.f ? q[2] / double(R.z) : 0.0;
-    }
-    scatter_grad(c.grad_mat, active ? mat : 0u, x, y, z, active);
+// R once).
+__device__ __forceinline__ void grad_material(GradCache& g, const PTConsts& c, uint32_t sid, uint32_t mat, V3 R) {
+    const double* q = c.qacc + 4ull * sid;
+    gc_add(g, c, mat, R.x != 0.f ? q[0] / double(R.x) : 0.0, R.y != 0.f ? q[1] / double(R.y) : 0.0,
+           R.z != 0.f ? q[2] / double(R.z) : 0.0);
 }
 
 // pt: pathtrace.cpp:203-262 for one segment.
+// The shade kernels walk the queue in tiles of kPTBlock paths, tile t by block t mod grid:
+// the render launches one block per tile, the gradient pass a device-filling grid so that
+// each block's gradient table collects many paths before it is flushed.
 template <int kMode>
 __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                        const float4* __restrict__ hits, Queue out,
                                                        uint32_t* out_count) {
-    const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
-    const bool valid = i < *in_count;
+    __shared__ GradCache gc;
+    if (kMode == kAdjGrad) gc_init(gc);
+    const uint32_t count = *in_count;
+    auto tile_body = [&](uint32_t tile) {
+    const uint32_t i = tile * kPTBlock + threadIdx.x;
+    const bool valid = i < count;
     const uint32_t j = valid ? i : 0u;  // invalid lanes read slot 0 and discard it
     const float4 ro = in.ray_o[j], rd = in.ray_d[j], th = in.thr[j], ax = in.aux[j];
     const uint4 meta = in.meta[j];
@@ -688,12 +709,19 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
         }
     }
     if (kMode == kAdjGrad) {
-        grad_emission(c, meta.w, Le_w, s.light, emits);
-        grad_material(c, meta.x, s.mat, m.R, alive && s.mat != kBlackMat);
+        if (emits) grad_emission(gc, c, meta.w, Le_w, s.light);
+        if (alive && s.mat != kBlackMat) grad_material(gc, c, meta.x, s.mat, m.R);
     }
     const uint32_t slot = block_append<0>(out_count, alive);
     if (alive) store_path(out, slot, s.p, kEps, wo, pdf, T, L, meta.x, uint32_t(n), rng.dim, meta.w);
     else if (kMode == kRender && valid) film_add(c, meta.w, L);
+    };
+    if (kMode == kAdjGrad) {
+        for (uint32_t tile = blockIdx.x; tile * kPTBlock < count; tile += gridDim.x) tile_body(tile);
+        gc_flush(gc, c);
+    } else if (blockIdx.x * kPTBlock < count) {
+        tile_body(blockIdx.x);
+    }
 }
 
 // pt.direct / pt.mis: the hit half of one loop iteration (pathtrace.direct.cpp:304-336,
@@ -703,8 +731,12 @@ template <int kMode>
 __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                         const float4* __restrict__ hits, Queue out,
                                                         uint32_t* out_count, ShadowQueue sq, uint32_t* shadow_count) {
-    const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
-    const bool valid = i < *in_count;
+    __shared__ GradCache gc;
+    if (kMode == kAdjGrad) gc_init(gc);
+    const uint32_t count = *in_count;
+    auto tile_body = [&](uint32_t tile) {
+    const uint32_t i = tile * kPTBlock + threadIdx.x;
+    const bool valid = i < count;
     const uint32_t j = valid ? i : 0u;  // invalid lanes read slot 0 and discard it
     const bool mis = c.renderer == LMGPU_PT_MIS;
     const float4 ro = in.ray_o[j], rd = in.ray_d[j], th = in.thr[j], ax = in.aux[j];
@@ -746,7 +778,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
         }
         if (emits) contribute<kMode>(c, meta.x, meta.w, Le_w * Le);
     }
-    if (kMode == kAdjGrad) grad_emission(c, meta.w, Le_w, s.light, emits);
+    if (kMode == kAdjGrad && emits) grad_emission(gc, c, meta.w, Le_w, s.light);
     if (alive) {
         n++;
         if (c.max_pv != -1 && n >= c.max_pv) alive = false;
@@ -774,11 +806,18 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
         if (is_zero(fs)) alive = false;
         else T = T * fs;
     }
-    if (kMode == kAdjGrad) grad_material(c, meta.x, s.mat, v.m.R, (rec.emit || alive) && s.mat != kBlackMat);
+    if (kMode == kAdjGrad && (rec.emit || alive) && s.mat != kBlackMat) grad_material(gc, c, meta.x, s.mat, v.m.R);
     const uint32_t slot = block_append<1>(out_count, alive);
     if (alive) {
         const uint32_t nf = uint32_t(n) | ((type & (1 << 2)) ? (kFlagSpecular << 16) : 0u);
         store_path(out, slot, s.p, kEps, wo, pdf, T, s.f.sn, meta.x, nf, rng.dim, meta.w);
+    }
+    };
+    if (kMode == kAdjGrad) {
+        for (uint32_t tile = blockIdx.x; tile * kPTBlock < count; tile += gridDim.x) tile_body(tile);
+        gc_flush(gc, c);
+    } else if (blockIdx.x * kPTBlock < count) {
+        tile_body(blockIdx.x);
     }
 }
 
@@ -824,16 +863,19 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue 
 template <int kMode>
 __global__ __launch_bounds__(kPTBlock) void k_shadow_apply(PTConsts c, ShadowQueue sq,
                                                            const uint32_t* __restrict__ count) {
-    const uint32_t i = blockIdx.x * kPTBlock + threadIdx.x;
-    const bool valid = i < *count;
-    const uint32_t j = valid ? i : 0u;
-    const bool visible = valid && sq.vis[j] != 0u;
-    const float4 cv = sq.c[j];
-    const uint32_t pixel = __float_as_uint(sq.d[j].w);
-    const int li = __float_as_int(cv.w);
-    const V3 w = v3(cv.x, cv.y, cv.z);
-    if (visible) contribute<kMode>(c, sq.sid[j], pixel, w * light_Le(c, li));
-    if (kMode == kAdjGrad) grad_emission(c, pixel, w, li, visible);
+    __shared__ GradCache gc;
+    if (kMode == kAdjGrad) gc_init(gc);
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x * kPTBlock + threadIdx.x; i < n; i += gridDim.x * kPTBlock) {
+        if (sq.vis[i] == 0u) continue;
+        const float4 cv = sq.c[i];
+        const uint32_t pixel = __float_as_uint(sq.d[i].w);
+        const int li = __float_as_int(cv.w);
+        const V3 w = v3(cv.x, cv.y, cv.z);
+        contribute<kMode>(c, sq.sid[i], pixel, w * light_Le(c, li));
+        if (kMode == kAdjGrad) grad_emission(gc, c, pixel, w, li);
+    }
+    if (kMode == kAdjGrad) gc_flush(gc, c);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -877,9 +919,9 @@ __global__ void k_resolve_comp(const float* __restrict__ acc, const float* __res
 // sum over samples of |Q.x| + |Q.y| + |Q.z| (the replay check of the adjoint).
 __global__ __launch_bounds__(256) void k_abs_sum(const double* __restrict__ q, uint32_t n, double* out) {
     __shared__ double part[256 / 64];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     double v = 0.0;
-    if (i < n) v = fabs(q[4ull * i]) + fabs(q[4ull * i + 1]) + fabs(q[4ull * i + 2]);
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        v += fabs(q[4ull * i]) + fabs(q[4ull * i + 1]) + fabs(q[4ull * i + 2]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
@@ -1010,6 +1052,9 @@ struct Pipeline {
         const dim3 grid_b((nb + kPTBlock - 1) / kPTBlock), grid_c((cap + kPTBlock - 1) / kPTBlock);
         // Persistent traversal grids: the device fill, never more than the queue capacity needs.
         const dim3 grid_t(std::min<uint32_t>(grid_c.x, trace_grid));
+        // The gradient pass: 4 blocks per CU, each flushing its gradient table once.
+        const dim3 grid_s(kMode == kAdjGrad ? std::min<uint32_t>(grid_c.x, 4u * uint32_t(std::max(ctx->num_cus, 1)))
+                                            : grid_c.x);
         hipError_t e = timed(ctx, kKGen, [&]() {
             hipLaunchKernelGGL(k_gen<kMode>, grid_b, dim3(kPTBlock), 0, s, c, b.q[0], b.counters + 0, b.sq,
                                b.counters + kMaxSegments + 0);
@@ -1022,7 +1067,7 @@ struct Pipeline {
             });
             if (e2 != hipSuccess) return hip_fail(ctx, e2, "k_shadow");
             if (kMode != kRender) {
-                hipLaunchKernelGGL(k_shadow_apply<kMode>, grid_c, dim3(kPTBlock), 0, s, c, b.sq,
+                hipLaunchKernelGGL(k_shadow_apply<kMode>, grid_s, dim3(kPTBlock), 0, s, c, b.sq,
                                    b.counters + kMaxSegments + k);
                 e2 = hipGetLastError();
                 if (e2 != hipSuccess) return hip_fail(ctx, e2, "k_shadow_apply");
@@ -1043,10 +1088,10 @@ struct Pipeline {
             if (e != hipSuccess) return hip_fail(ctx, e, "k_extend");
             e = timed(ctx, kKShade, [&]() {
                 if (is_pt)
-                    hipLaunchKernelGGL(k_shade_pt<kMode>, grid_c, dim3(kPTBlock), 0, s, c, qi, b.counters + k, b.hits,
+                    hipLaunchKernelGGL(k_shade_pt<kMode>, grid_s, dim3(kPTBlock), 0, s, c, qi, b.counters + k, b.hits,
                                        qo, b.counters + k + 1);
                 else
-                    hipLaunchKernelGGL(k_shade_nee<kMode>, grid_c, dim3(kPTBlock), 0, s, c, qi, b.counters + k,
+                    hipLaunchKernelGGL(k_shade_nee<kMode>, grid_s, dim3(kPTBlock), 0, s, c, qi, b.counters + k,
                                        b.hits, qo, b.counters + k + 1, b.sq, b.counters + kMaxSegments + k + 1);
             });
             if (e != hipSuccess) return hip_fail(ctx, e, "k_shade");
@@ -1210,10 +1255,11 @@ lmgpu_status run_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_adj
                            reinterpret_cast<uint4*>(pl.c.qacc), size_t(nb) * 2);
         st = pl.batch<kAdjTotal>(p->sample_offset + base, nb);
         if (st != LMGPU_OK) break;
-        hipLaunchKernelGGL(k_abs_sum, dim3((nb + 255) / 256), dim3(256), 0, s, pl.c.qacc, nb, sums + 0);
+        const dim3 grid_sum(std::min<uint32_t>((nb + 255) / 256, 256u));
+        hipLaunchKernelGGL(k_abs_sum, grid_sum, dim3(256), 0, s, pl.c.qacc, nb, sums + 0);
         st = pl.batch<kAdjGrad>(p->sample_offset + base, nb);
         if (st != LMGPU_OK) break;
-        hipLaunchKernelGGL(k_abs_sum, dim3((nb + 255) / 256), dim3(256), 0, s, pl.c.qacc, nb, sums + 1);
+        hipLaunchKernelGGL(k_abs_sum, grid_sum, dim3(256), 0, s, pl.c.qacc, nb, sums + 1);
     }
     if (st == LMGPU_OK) {
         // The film is Sum * W*H/N (sched.mt.cpp:256): the same factor scales its gradient.

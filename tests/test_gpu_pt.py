@@ -177,3 +177,47 @@ def test_error_paths(gpu):
     assert e.value.status == 8
     film, st = gpu.render("pt")
     assert np.isfinite(film).all() and st.samples == 256
+
+
+@pytest.fixture(scope="module")
+def terrain(oracle):
+    """The 2M-triangle terrain at a reduced film, with its FP64 oracle (BVH built once)."""
+    sc = scenes.terrain_scene(240, 135, 4)
+    return sc, oracle.Oracle(sc, fp64=True, accel="bvh")
+
+
+@pytest.mark.parametrize("renderer", ["pt", "pt.mis"])
+def test_terrain_same_streams(gpu, terrain, renderer):
+    """2M triangles, 40 % mirrors: after a few mirror bounces on the bumpy surface FP32 and
+    FP64 paths drift apart more often than in the room, so the bar is looser."""
+    sc, o = terrain
+    gpu.upload(sc)
+    gpu.build_bvh()
+    g, _ = gpu.render(renderer, film_precision="fp64")
+    ref, _ = o.render(renderer, threads=0)
+    rel_total = abs(g.sum() - ref.sum()) / ref.sum()
+    close = np.abs(g - ref) <= 1e-3 * np.maximum(np.abs(ref), 1e-2 * np.abs(ref).mean())
+    print(f"terrain/{renderer}: total rel diff {rel_total:.2e}, values within 1e-3: {close.mean():.5f}", flush=True)
+    assert rel_total < 2e-2
+    assert close.mean() > 0.9
+
+
+def test_terrain_within_3_sigma(gpu):
+    """Independent streams on the terrain: per-pixel means of 16 GPU and 16 oracle renders
+    (96 x 54, 64 spp each) agree within the combined Monte Carlo error."""
+    sc = scenes.terrain_scene(96, 54, 64)
+    gpu.upload(sc)
+    gpu.build_bvh()
+    from oracle import lmref
+
+    o = lmref.Oracle(sc, fp64=True, accel="bvh")
+    runs_g = np.array([gpu.render("pt.mis", seed=300 + s, film_precision="fp64")[0] for s in range(16)])
+    runs_o = np.array([o.render("pt.mis", seed=400 + s, threads=0)[0] for s in range(16)])
+    mg, mo = runs_g.mean(0), runs_o.mean(0)
+    se = np.sqrt(runs_g.var(0, ddof=1) / 16 + runs_o.var(0, ddof=1) / 16 + 1e-12)
+    z = (mg - mo) / se
+    print(f"terrain 3-sigma: frac |z|>3 {np.mean(np.abs(z) > 3):.4f}, mean z^2 {np.mean(z**2):.3f}, "
+          f"mean z {z.mean():.3f}", flush=True)
+    assert np.mean(np.abs(z) > 3.5) < 0.02
+    assert np.mean(z ** 2) < 2.0
+    assert abs(z.mean()) < 0.3
