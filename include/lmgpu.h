@@ -24,6 +24,8 @@
  *                        (src/liblightmetrica/pathtrace.cpp:165-266, pathtrace.direct.cpp:170-338,
  *                        pathtrace.mis.cpp:167-369) and HDRBitmapFilm::AccumulateContribution + Rescale
  *                        (src/liblightmetrica/hdrfilm.cpp:151-171, :334-341)
+ *   lmgpu_raycast_render RaycastRenderer::ProcessSinglePixel (src/liblightmetrica/raycast.cpp:96-137) over every
+ *                        pixel, as DeterministicMTRenderProcessScheduler::Render (sched.pixel.cpp:79-126)
  *   lmgpu_pt_backward    no counterpart in the reference: the path-replay adjoint of lmgpu_pt_render
  *                        with respect to material reflectances and light emissions
  *   lmgpu_scene_set_params  the `diffuse_reflectance` / `specular_reflectance` and `luminance` parameters
@@ -45,7 +47,7 @@
 extern "C" {
 #endif
 
-#define LMGPU_ABI_VERSION 2
+#define LMGPU_ABI_VERSION 3
 
 typedef int lmgpu_status;
 #define LMGPU_OK 0
@@ -73,7 +75,8 @@ typedef struct lmgpu_material {
 /* Lights -- AreaLight (src/liblightmetrica/light.area.cpp), ConstantEnvironmentLight
  * (light.env.const.cpp).  The order is the light-selection order of
  * Scene::SampleLightSelection (src/liblightmetrica/scene.cpp:79-86); the reference
- * puts the environment light first (src/liblightmetrica/primitives.cpp:140). */
+ * appends the environment light after the area lights of the scene tree
+ * (src/liblightmetrica/primitives.cpp:122-141). */
 #define LMGPU_LIGHT_AREA 0u
 #define LMGPU_LIGHT_ENV_CONST 1u
 typedef struct lmgpu_light {
@@ -217,6 +220,9 @@ lmgpu_status lmgpu_trace_count(lmgpu_ctx* ctx, const lmgpu_rays_soa* rays, uint3
 
 /* ---- path tracing ------------------------------------------------------------- */
 lmgpu_status lmgpu_pt_render(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_film* film, lmgpu_pt_stats* stats);
+/* Deterministic raycast image: |shading normal . -d| at each pixel centre, 0 on a miss
+ * (RecordContribution, not accumulated); bit-identical from run to run. */
+lmgpu_status lmgpu_raycast_render(lmgpu_ctx* ctx, lmgpu_film* film);
 /* Path-replay adjoint; film_precision is ignored (gradients reduce in FP64). */
 lmgpu_status lmgpu_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_adjoint* adjoint,
                                lmgpu_pt_stats* stats);

@@ -25,7 +25,8 @@ LMGPU_FILM_FP32, LMGPU_FILM_FP64, LMGPU_FILM_FP32_COMPENSATED = 0, 1, 2
 EXPORTS = [
     "lmgpu_create", "lmgpu_destroy", "lmgpu_last_error", "lmgpu_abi_version", "lmgpu_set_stream",
     "lmgpu_synchronize", "lmgpu_scene_upload", "lmgpu_bvh_build", "lmgpu_trace_closest", "lmgpu_trace_any",
-    "lmgpu_trace_count", "lmgpu_pt_render", "lmgpu_pt_backward", "lmgpu_scene_set_params", "lmgpu_enable_timing",
+    "lmgpu_trace_count", "lmgpu_pt_render", "lmgpu_pt_backward", "lmgpu_scene_set_params",
+    "lmgpu_raycast_render", "lmgpu_enable_timing",
     "lmgpu_kernel_timing", "lmgpu_reset_timing", "lmgpu_inject_alloc_failures",
 ]
 
@@ -136,6 +137,7 @@ def load() -> C.CDLL:
     lib.lmgpu_pt_render.argtypes = [vp, C.POINTER(PtParams), C.POINTER(Film), C.POINTER(PtStats)]
     lib.lmgpu_pt_backward.argtypes = [vp, C.POINTER(PtParams), C.POINTER(Adjoint), C.POINTER(PtStats)]
     lib.lmgpu_scene_set_params.argtypes = [vp, C.c_void_p, C.c_void_p]
+    lib.lmgpu_raycast_render.argtypes = [vp, C.POINTER(Film)]
     lib.lmgpu_enable_timing.argtypes = [vp, C.c_int]
     lib.lmgpu_kernel_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     lib.lmgpu_reset_timing.argtypes = [vp]

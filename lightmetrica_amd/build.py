@@ -92,10 +92,23 @@ def build_driver() -> Path:
     return DRIVER
 
 
+BVH_CHECK = ROOT / "tests" / "native" / "bvh_check"
+
+
+def build_bvh_check() -> Path:
+    """The host-side BVH invariant checker (builder source compiled in, no GPU)."""
+    tmp = BVH_CHECK.with_name(BVH_CHECK.name + ".tmp")
+    _run(["g++", *HOST_FLAGS, str(ROOT / "tests" / "native" / "bvh_check.cpp"), str(CSRC / "bvh_build.cpp"), "-o",
+          str(tmp), "-lpthread"])
+    os.replace(tmp, BVH_CHECK)
+    return BVH_CHECK
+
+
 def build_all(jobs: int = 4) -> None:
     build_lib(jobs)
     build_plugin()
     build_driver()
+    build_bvh_check()
 
 
 if __name__ == "__main__":

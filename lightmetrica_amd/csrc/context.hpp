@@ -172,5 +172,6 @@ hipError_t launch_trace_any(lmgpu_ctx* ctx, const float* const* rays, uint32_t n
 hipError_t launch_trace_count(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, unsigned long long* counts);
 lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, lmgpu_pt_stats* stats);
 lmgpu_status run_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_adjoint* adj, lmgpu_pt_stats* stats);
+lmgpu_status run_raycast(lmgpu_ctx* ctx, lmgpu_film* film);
 
 }  // namespace lmgpu

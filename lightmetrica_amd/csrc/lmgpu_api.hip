@@ -584,6 +584,12 @@ lmgpu_status lmgpu_pt_render(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgp
     return run_pt(ctx, params, film, stats);
 }
 
+lmgpu_status lmgpu_raycast_render(lmgpu_ctx* ctx, lmgpu_film* film) {
+    if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    return run_raycast(ctx, film);
+}
+
 lmgpu_status lmgpu_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* params, lmgpu_adjoint* adjoint,
                                lmgpu_pt_stats* stats) {
     if (!ctx) return LMGPU_ERR_INVALID_ARGUMENT;

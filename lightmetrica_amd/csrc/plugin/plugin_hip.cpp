@@ -6,8 +6,10 @@
 //   renderer    "pt.hip"          -- PathtraceRenderer configuration (pathtrace.cpp:122-154), estimator
 //               "pt.direct.hip"      pt / pt.direct / pt.mis
 //               "pt.mis.hip"
+//               "raycast.hip"     -- RaycastRenderer (raycast.cpp:96-137): |sn . -d| at pixel centres
 //   rendersched "hip"             -- the batched scheduler: renders num_samples on the GPU and leaves
-//                                    the rescaled image in the camera's film (sched.mt.cpp:93-266)
+//                                    the rescaled image in the camera's film (sched.mt.cpp:93-266); for
+//                                    raycast.hip, one deterministic pass over the pixels (sched.pixel.cpp:79-126)
 //
 // Scene content reaches the GPU through the assets' public virtuals only:
 //   diffuse R       = pi * EvaluateDirection at normal incidence        (diffuse.cpp:192-208)
@@ -377,6 +379,19 @@ class HIPMISPathtraceRenderer final : public HIPPathtraceRenderer {
     std::string DefaultEstimator() const override { return "pt.mis"; }
 };
 
+class HIPRaycastRenderer final : public Renderer {
+  public:
+    LM_COMPONENT_IMPL_DEF("raycast.hip");
+    std::string Type() const override { return ComponentImplTypeName(); }
+    bool Configure(const ConfigNode&, const Assets&, const Scene&, const RenderProcessScheduler&) override { return true; }
+    bool Preprocess(const Scene&, const RenderProcessScheduler&) override { return true; }
+    bool Postprocess(const Scene&, const RenderProcessScheduler&) const override { return true; }
+    RenderProcess* CreateRenderProcess(const Scene&, int, int) override {
+        log_error("raycast.hip renders only under the 'hip' render scheduler");
+        return nullptr;
+    }
+};
+
 class HIPRenderProcessScheduler final : public SamplingBasedRenderProcessScheduler {
   public:
     LM_COMPONENT_IMPL_DEF("hip");
@@ -422,7 +437,8 @@ class HIPRenderProcessScheduler final : public SamplingBasedRenderProcessSchedul
         }
         const HIPScene* hs = dynamic_cast<const HIPScene*>(&scene);
         HIPPathtraceRenderer* hr = dynamic_cast<HIPPathtraceRenderer*>(&renderer);
-        if (!hs || !hr) {
+        HIPRaycastRenderer* rc = dynamic_cast<HIPRaycastRenderer*>(&renderer);
+        if (!hs || (!hr && !rc)) {
             log_error("hip scheduler needs the 'hip.wbvh' scene and a '*.hip' renderer");
             return false;
         }
@@ -433,6 +449,20 @@ class HIPRenderProcessScheduler final : public SamplingBasedRenderProcessSchedul
             return false;
         }
         if (report_) report_(0, false);
+        if (rc) {
+            const int W = film->Width(), H = film->Height();
+            std::vector<double> img(size_t(W) * size_t(H) * 3);
+            lmgpu_film out{img.data(), 0, LMGPU_FILM_FP64, uint32_t(W), uint32_t(H)};
+            if (lmgpu_raycast_render(hs->Context(), &out) != LMGPU_OK) {
+                log_error(std::string("hip scheduler: ") + lmgpu_last_error(hs->Context()));
+                return false;
+            }
+            auto& data = film->Bitmap().InternalData();
+            data.resize(img.size());
+            for (size_t k = 0; k < img.size(); k++) data[k] = Float(img[k]);
+            if (report_) report_(1, true);
+            return true;
+        }
         lmgpu_pt_params p{};
         p.renderer = hr->Estimator();
         p.num_samples = uint64_t(num_samples_);
@@ -495,6 +525,7 @@ const Register<HIPScene, Scene> reg_scene;
 const Register<HIPPathtraceRenderer, Renderer> reg_pt;
 const Register<HIPDirectPathtraceRenderer, Renderer> reg_pt_direct;
 const Register<HIPMISPathtraceRenderer, Renderer> reg_pt_mis;
+const Register<HIPRaycastRenderer, Renderer> reg_raycast;
 const Register<HIPRenderProcessScheduler, RenderProcessScheduler> reg_sched;
 
 }  // namespace

@@ -878,6 +878,51 @@ __global__ __launch_bounds__(kPTBlock) void k_shadow_apply(PTConsts c, ShadowQue
     if (kMode == kAdjGrad) gc_flush(gc, c);
 }
 
+// The raycast renderer (RaycastRenderer_RenderProcess::ProcessSinglePixel,
+// src/liblightmetrica/raycast.cpp:96-137, under the deterministic per-pixel scheduler,
+// sched.pixel.cpp:79-126): the ray through each pixel centre, |sn . -d| on a hit --
+// triangles first, then the environment sphere -- and black otherwise.  Every pixel is a
+// function of its own ray only, so the image is bit-identical from run to run.
+struct RaycastSource {
+    CameraDev cam;
+    __device__ __forceinline__ RayIn operator()(uint32_t i) const {
+        const uint32_t x = i % uint32_t(cam.width), y = i / uint32_t(cam.width);
+        float pdf;
+        const V3 d = cam_direction(cam, (0.5f + float(x)) / float(cam.width), (0.5f + float(y)) / float(cam.height), pdf);
+        return RayIn{cam.pos[0], cam.pos[1], cam.pos[2], d.x, d.y, d.z, 0.f, kInf};
+    }
+};
+
+template <class Out>
+struct RaycastSink {
+    PTConsts c;
+    Out* out;
+    __device__ __forceinline__ void operator()(uint32_t i, bool found, const HitOut& h, const RayIn& r) const {
+        const V3 o = v3(r.ox, r.oy, r.oz), d = v3(r.dx, r.dy, r.dz);
+        float4 hit = make_float4(0.f, 0.f, 0.f, __uint_as_float(kMiss));
+        if (found) {
+            float t, b0, b1;
+            winner_barycentrics(c.tris, r, h, false, t, b0, b1);
+            hit = make_float4(t, b0, b1, __uint_as_float(h.prim));
+        }
+        Surf s;
+        float v = 0.f;
+        if (surface_at(c, o, d, 0.f, hit, s)) v = fabsf(dot(s.f.sn, -d));
+        out[3ull * i + 0] = Out(v);
+        out[3ull * i + 1] = Out(v);
+        out[3ull * i + 2] = Out(v);
+    }
+};
+
+template <class Out>
+__global__ __launch_bounds__(kTraceBlock) void k_raycast(PTConsts c, uint32_t n, uint32_t* next, Out* out) {
+    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    RaycastSource src{c.cam};
+    RaycastSink<Out> sink{c, out};
+    TraceCounters cnt{0, 0};
+    trace_persistent<false, false>(c.nodes, c.tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -1108,6 +1153,57 @@ struct Pipeline {
         return LMGPU_OK;
     }
 };
+
+lmgpu_status run_raycast(lmgpu_ctx* ctx, lmgpu_film* film) {
+    if (!ctx->has_bvh) return fail(ctx, LMGPU_ERR_NO_BVH, "lmgpu_raycast_render: build the BVH first");
+    if (!ctx->has_shading || !ctx->has_camera)
+        return fail(ctx, LMGPU_ERR_NO_SCENE, "lmgpu_raycast_render: scene has no materials or camera");
+    if (!film || !film->data || film->width != uint32_t(ctx->cam.width) || film->height != uint32_t(ctx->cam.height))
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_raycast_render: film size differs from the camera's");
+    if (film->precision != LMGPU_FILM_FP32 && film->precision != LMGPU_FILM_FP64)
+        return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_raycast_render: output film precision");
+    if (3u * ctx->bvh_max_depth > uint32_t(kStackTotal))
+        return fail(ctx, LMGPU_ERR_BVH_TOO_DEEP, "lmgpu_raycast_render: BVH deeper than the traversal stack");
+    const uint32_t n = uint32_t(ctx->cam.width) * uint32_t(ctx->cam.height);
+    const size_t out_bytes = size_t(n) * 3 * (film->precision == LMGPU_FILM_FP64 ? 8 : 4);
+    void* out_dev = film->data;
+    if (!film->device) {
+        lmgpu_status st = ensure(ctx, &ctx->film_out_staging, &ctx->film_out_staging_bytes, out_bytes, "film staging");
+        if (st != LMGPU_OK) return st;
+        out_dev = ctx->film_out_staging;
+    }
+    PTConsts c{};
+    c.nodes = ctx->nodes.ptr;
+    c.tris = ctx->tris.ptr;
+    c.shade = ctx->shade.ptr;
+    c.mats = ctx->mats.ptr;
+    c.lights = ctx->lights.ptr;
+    c.cam = ctx->cam;
+    c.env = ctx->env;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_raycast<float>, kTraceBlock, 0) != hipSuccess || occ < 1)
+        occ = 1;
+    const uint32_t grid = std::min<uint32_t>((n + kTraceBlock - 1) / kTraceBlock,
+                                             uint32_t(std::max(ctx->num_cus, 1)) * uint32_t(occ));
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctx->next_counter, 1u);
+    hipError_t e = timed(ctx, kKTraceClosest, [&]() {
+        if (film->precision == LMGPU_FILM_FP64)
+            hipLaunchKernelGGL(k_raycast<double>, dim3(grid), dim3(kTraceBlock), 0, s, c, n, ctx->next_counter,
+                               static_cast<double*>(out_dev));
+        else
+            hipLaunchKernelGGL(k_raycast<float>, dim3(grid), dim3(kTraceBlock), 0, s, c, n, ctx->next_counter,
+                               static_cast<float*>(out_dev));
+    });
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_raycast");
+    if (!film->device) {
+        e = hipMemcpyAsync(film->data, out_dev, out_bytes, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "raycast readback");
+    }
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "raycast synchronize");
+    return LMGPU_OK;
+}
 
 lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, lmgpu_pt_stats* stats) {
     lmgpu_status st = validate(ctx, p, "lmgpu_pt_render");

@@ -225,6 +225,20 @@ class Device:
         return out, RenderStats(int(stats.closest_rays), int(stats.shadow_rays), int(stats.samples),
                                 float(stats.seconds))
 
+    def raycast(self, out_precision: str = "fp32") -> np.ndarray:
+        """Deterministic raycast image (raycast renderer): |sn . -d| at pixel centres, (H, W, 3)."""
+        sc = self.scene
+        if sc is None or sc.camera is None:
+            raise LmgpuError(3, "upload a scene with a camera first")
+        W, H = sc.camera.width, sc.camera.height
+        out = np.empty((H, W, 3), dtype=np.float64 if out_precision == "fp64" else np.float32)
+        film = N.Film()
+        film.data, film.device = out.ctypes.data, 0
+        film.precision = N.LMGPU_FILM_FP64 if out_precision == "fp64" else N.LMGPU_FILM_FP32
+        film.width, film.height = W, H
+        self._check(self._lib.lmgpu_raycast_render(self._ctx, C.byref(film)))
+        return out
+
     def set_params(self, albedo=None, emission=None) -> None:
         """Replace material reflectances ((num_materials, 3)) and / or light emissions
         ((num_lights, 3)) of the uploaded scene in place."""

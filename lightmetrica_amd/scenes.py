@@ -316,7 +316,8 @@ def terrain_scene(width: int = 1920, height: int = 1080, spp: int = 64, grid: in
     (6 octaves of gradient noise from 4 cells, lacunarity 2, gain 0.5), 2 grid^2 triangles
     with interpolated normals.  Per triangle: a mirror (R = 1) with probability 0.4, else a
     diffuse material of its own with reflectance U[0.1, 0.9]^3.  A 1 x 1 quad light at
-    y = 1.5 facing down (Le = 10) and a constant environment (Le = 0.5, first light)."""
+    y = 1.5 facing down (Le = 10) and a constant environment (Le = 0.5), in the reference's
+    light order (area lights, then the environment light)."""
     n = grid + 1
     c = np.arange(n, dtype=np.float64) / grid
     X, Z = np.meshgrid(c, c, indexing="ij")
@@ -359,8 +360,8 @@ def terrain_scene(width: int = 1920, height: int = 1080, spp: int = 64, grid: in
     Nn = np.concatenate([Nv.reshape(-1, 3), np.tile([0.0, -1.0, 0.0], (4, 1))]).astype(np.float32)
     F = np.concatenate([F, [[base, base + 1, base + 2], [base, base + 2, base + 3]]]).astype(np.uint32)
     M = np.concatenate([M, [0, 0]]).astype(np.uint32)
-    L = np.concatenate([np.full(nt, -1), [1, 1]]).astype(np.int32)
-    lights = _lights([(LIGHT_ENV_CONST, (0.5, 0.5, 0.5)), (LIGHT_AREA, (10.0, 10.0, 10.0))])
+    L = np.concatenate([np.full(nt, -1), [0, 0]]).astype(np.int32)
+    lights = _lights([(LIGHT_AREA, (10.0, 10.0, 10.0)), (LIGHT_ENV_CONST, (0.5, 0.5, 0.5))])
     view = look_at((0.5, 0.6, 1.6), (0.5, 0.1, 0.5), (0.0, 1.0, 0.0)).astype(np.float32)
     cam = Camera(view=view, fovy=45.0, width=width, height=height)
     rc = RenderConfig(num_samples=width * height * spp, max_path_vertices=17, rr_depth=3, seed=1)

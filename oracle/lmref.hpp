@@ -1307,6 +1307,24 @@ template <class F> struct Film {
 // Estimators.  `rays` counts closest-hit queries actually traced (including the
 // shadow queries of Visible()).
 
+// RaycastRenderer_RenderProcess::ProcessSinglePixel (src/liblightmetrica/raycast.cpp:96-137)
+// under DeterministicMTRenderProcessScheduler (sched.pixel.cpp:79-126): the ray through the
+// pixel centre, value |sn . -d| on a hit (triangles, then emitter shapes), black otherwise,
+// recorded (not accumulated) into the pixel.
+template <class F> void raycast_pixel(const Scene<F>& sc, int x, int y, int W, int H, F out[3]) {
+    const V2<F> raster((F(0.5) + F(x)) / F(W), (F(0.5) + F(y)) / F(H));
+    F pdf;
+    Ray<F> ray;
+    ray.o = sc.cam.position;
+    ray.d = sc.cam.sample_direction(raster, pdf);
+    ray.minT = F(0);
+    ray.maxT = K<F>::inf();
+    Isect<F> is;
+    F c = F(0);
+    if (scene_intersect(sc, ray, is)) c = std::abs(dot(is.g.sn, -ray.d));
+    out[0] = out[1] = out[2] = c;
+}
+
 struct RenderParams {
     int max_path_vertices = -1;  // pathtrace.cpp:126
     int rr_depth = 1;            // pathtrace.cpp:125

@@ -62,6 +62,7 @@ def load() -> C.CDLL:
     lib.lmref_intersect_geom.argtypes = [vp, fp, fp, C.c_float, dp]
     lib.lmref_render.argtypes = [vp, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
                                  C.c_longlong, C.c_uint64, dp, dp]
+    lib.lmref_raycast.argtypes = [vp, C.c_int, dp]
     lib.lmref_philox.argtypes = [u32p, u32p, u32p]
     lib.lmref_philox_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p]
     lib.lmref_concentric_disk.argtypes = [C.c_int, dp, dp]
@@ -185,6 +186,19 @@ class Oracle:
         return film.reshape(H, W, 3), {"rays": int(stats[0]), "seconds": float(stats[1]),
                                        "dropped": int(stats[2]), "samples": int(stats[3]),
                                        "threads": int(stats[4])}
+
+
+def _raycast(self, threads: int = 0) -> np.ndarray:
+    """The raycast renderer (raycast.cpp:96-137): |sn . -d| at each pixel centre."""
+    W, H = self.scene.camera.width, self.scene.camera.height
+    film = np.zeros(W * H * 3)
+    st = self.lib.lmref_raycast(self.h, threads, film.ctypes.data_as(C.POINTER(C.c_double)))
+    if st != 0:
+        raise RuntimeError(f"lmref_raycast failed ({st})")
+    return film.reshape(H, W, 3)
+
+
+Oracle.raycast = _raycast
 
 
 def philox(key, ctr):

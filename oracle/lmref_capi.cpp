@@ -228,6 +228,25 @@ int lmref_intersect_geom(void* p, const float* o, const float* d, float tmin, do
     return h->fp64 ? run(*h->s64) : run(*h->s32);
 }
 
+// The raycast renderer over all pixels (OpenMP over rows, like det.mt).  film_out: W*H*3.
+int lmref_raycast(void* p, int nthreads, double* film_out) {
+    SceneHandle* h = static_cast<SceneHandle*>(p);
+    auto run = [&](auto& sc) {
+        using F = std::decay_t<decltype(sc.soup.p1[0].x)>;
+        const int W = sc.cam.width, H = sc.cam.height;
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++) {
+                F c[3];
+                raycast_pixel(sc, x, y, W, H, c);
+                for (int k = 0; k < 3; k++) film_out[3 * (size_t(y) * W + x) + k] = double(c[k]);
+            }
+        return 0;
+    };
+    return h->fp64 ? run(*h->s64) : run(*h->s32);
+}
+
 int lmref_render(void* p, int kind, uint64_t num_samples, int max_path_vertices, int rr_depth, uint64_t seed, int rng,
                  int nthreads, long long samples_per_block, uint64_t sample_offset, double* film_out, double* stats) {
     SceneHandle* h = static_cast<SceneHandle*>(p);

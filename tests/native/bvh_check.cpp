@@ -1,0 +1,210 @@
+// Host-side invariants of the wide-BVH builder (no GPU): built from
+// lightmetrica_amd/csrc/bvh_build.cpp and run by tests/test_bvh_builder.py.
+//
+//   bvh_check <num_triangles> <seed> <max_leaf>
+//
+// Checks, on a random triangle soup (uniform centroids, edges up to 0.05):
+//   1. every triangle is referenced by exactly one leaf record;
+//   2. every quantized child box, dequantized exactly, contains the exact child box and
+//      every triangle below that child;
+//   3. the device slab test restated in float (same operations as traverse.hpp) never
+//      rejects a child whose exact box a random ray enters, for rays from inside and
+//      around the scene (conservativeness of the quantized planes and the robust test);
+//   4. nor a ray aimed at a corner or an edge point of the exact child box (grazing rays).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../lightmetrica_amd/csrc/bvh_build.h"
+
+using namespace lmgpu;
+
+static int failures = 0;
+#define CHECK(c, ...)                                   \
+    do {                                                \
+        if (!(c)) {                                     \
+            if (failures < 20) std::printf(__VA_ARGS__); \
+            failures++;                                 \
+        }                                               \
+    } while (0)
+
+static double plane_exact(const Node4& n, int axis, uint32_t word, int i) {
+    const uint32_t q = (word >> (8 * i)) & 0xffu;
+    const int e = int((n.exps >> (8 * axis)) & 0xffu) - 127;
+    return double(n.origin[axis]) + std::ldexp(double(q), e);
+}
+
+struct Box {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float* p) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], double(p[a]));
+            hi[a] = std::max(hi[a], double(p[a]));
+        }
+    }
+};
+
+// Exact triangle bounds below a child reference.
+static Box subtree_box(const WideBVH& w, uint32_t ref, std::vector<int>& seen) {
+    Box b;
+    if (ref & kLeafBit) {
+        const uint32_t first = ref & kMaxLeafFirst, count = ((ref >> 27) & 0xfu) + 1u;
+        for (uint32_t j = first; j < first + count; j++) {
+            const TriRecord& t = w.tris[j];
+            b.grow(t.v0);
+            b.grow(t.v1);
+            b.grow(t.v2);
+            uint32_t id;
+            std::memcpy(&id, &t.v0[3], 4);
+            seen[id]++;
+        }
+        return b;
+    }
+    const Node4& n = w.nodes[ref];
+    const uint32_t lo[3] = {n.q_lo_x, n.q_lo_y, n.q_lo_z}, hi[3] = {n.q_hi_x, n.q_hi_y, n.q_hi_z};
+    for (int i = 0; i < 4; i++) {
+        if (n.child[i] == kEmptyChild) continue;
+        Box c = subtree_box(w, n.child[i], seen);
+        const BoxNode4& bn = w.box_nodes[ref];
+        const float* blo[3] = {bn.lo_x, bn.lo_y, bn.lo_z};
+        const float* bhi[3] = {bn.hi_x, bn.hi_y, bn.hi_z};
+        for (int a = 0; a < 3; a++) {
+            CHECK(plane_exact(n, a, lo[a], i) <= c.lo[a] && plane_exact(n, a, hi[a], i) >= c.hi[a],
+                  "node %u child %d axis %d: quantized [%g, %g] does not contain [%g, %g]\n", ref, i, a,
+                  plane_exact(n, a, lo[a], i), plane_exact(n, a, hi[a], i), c.lo[a], c.hi[a]);
+            CHECK(plane_exact(n, a, lo[a], i) <= blo[a][i] && plane_exact(n, a, hi[a], i) >= bhi[a][i],
+                  "node %u child %d axis %d: quantized box does not contain the exact child box\n", ref, i, a);
+            b.lo[a] = std::min(b.lo[a], c.lo[a]);
+            b.hi[a] = std::max(b.hi[a], c.hi[a]);
+        }
+    }
+    return b;
+}
+
+static float safe_rcp(float d) { return 1.f / (std::fabs(d) > 1e-30f ? d : std::copysign(1e-30f, d)); }
+
+int main(int argc, char** argv) {
+    const uint32_t nt = argc > 1 ? uint32_t(std::atoi(argv[1])) : 20000;
+    const uint32_t seed = argc > 2 ? uint32_t(std::atoi(argv[2])) : 1;
+    const uint32_t leaf = argc > 3 ? uint32_t(std::atoi(argv[3])) : 4;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    std::vector<float> P(9 * size_t(nt));
+    std::vector<uint32_t> F(3 * size_t(nt));
+    for (uint32_t t = 0; t < nt; t++) {
+        const float c[3] = {U(rng), U(rng), U(rng)};
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++) P[9 * t + 3 * k + a] = c[a] + 0.05f * (U(rng) - 0.5f);
+        for (int k = 0; k < 3; k++) F[3 * t + k] = 3 * t + k;
+    }
+    BuildParams bp;
+    bp.max_leaf_size = leaf;
+    WideBVH w;
+    std::string err;
+    if (!build_wide_bvh(P.data(), 3 * nt, F.data(), nt, bp, w, err)) {
+        std::printf("build failed: %s\n", err.c_str());
+        return 2;
+    }
+    std::vector<int> seen(nt, 0);
+    subtree_box(w, 0, seen);
+    for (uint32_t t = 0; t < nt; t++) CHECK(seen[t] == 1, "triangle %u referenced %d times\n", t, seen[t]);
+
+    // 3. Ray / child-box tests: float restatement of node_children against the exact boxes.
+    const float kBoxRobust = 1.0f + 2.0f * 1.8e-7f;
+    uint64_t tested = 0, entered = 0;
+    for (int r = 0; r < 20000; r++) {
+        const float o[3] = {-0.5f + 2.f * U(rng), -0.5f + 2.f * U(rng), -0.5f + 2.f * U(rng)};
+        const float z = 1.f - 2.f * U(rng), ph = 6.2831853f * U(rng), s = std::sqrt(std::max(0.f, 1.f - z * z));
+        const float d[3] = {s * std::cos(ph), s * std::sin(ph), z};
+        const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
+        for (size_t ni = 0; ni < w.nodes.size(); ni += 7) {
+            const Node4& n = w.nodes[ni];
+            const BoxNode4& bn = w.box_nodes[ni];
+            float a[3], b[3];
+            for (int k = 0; k < 3; k++) {
+                float sc;
+                const uint32_t bits = ((n.exps >> (8 * k)) & 0xffu) << 23;
+                std::memcpy(&sc, &bits, 4);
+                a[k] = sc * inv[k];
+                b[k] = (n.origin[k] - o[k]) * inv[k];
+            }
+            const uint32_t lo[3] = {n.q_lo_x, n.q_lo_y, n.q_lo_z}, hi[3] = {n.q_hi_x, n.q_hi_y, n.q_hi_z};
+            const float* blo[3] = {bn.lo_x, bn.lo_y, bn.lo_z};
+            const float* bhi[3] = {bn.hi_x, bn.hi_y, bn.hi_z};
+            for (int i = 0; i < 4; i++) {
+                if (n.child[i] == kEmptyChild) continue;
+                float tn = 0.f, tf = 3.4e38f;
+                double etn = 0.0, etf = INFINITY;
+                for (int k = 0; k < 3; k++) {
+                    const float t0 = std::fma(float((lo[k] >> (8 * i)) & 0xffu), a[k], b[k]);
+                    const float t1 = std::fma(float((hi[k] >> (8 * i)) & 0xffu), a[k], b[k]);
+                    tn = std::max(tn, std::min(t0, t1));
+                    tf = std::min(tf, std::max(t0, t1));
+                    // exact entry / exit of the unquantized child box
+                    const double dd = double(d[k]);
+                    if (dd == 0.0) {
+                        if (o[k] < blo[k][i] || o[k] > bhi[k][i]) etf = -1.0;
+                        continue;
+                    }
+                    const double e0 = (double(blo[k][i]) - o[k]) / dd, e1 = (double(bhi[k][i]) - o[k]) / dd;
+                    etn = std::max(etn, std::min(e0, e1));
+                    etf = std::min(etf, std::max(e0, e1));
+                }
+                tested++;
+                if (etn <= etf) {
+                    entered++;
+                    CHECK(tn <= tf * kBoxRobust, "ray %d node %zu child %d: entered exactly [%g, %g] but rejected [%g, %g]\n",
+                          r, ni, i, etn, etf, double(tn), double(tf));
+                }
+            }
+        }
+    }
+    // 4. Grazing rays: aimed from random origins at corners and edge points of the exact
+    //    child boxes (the rays that only touch the box), which the test must still accept.
+    uint64_t grazing = 0;
+    for (size_t ni = 0; ni < w.nodes.size(); ni++) {
+        const Node4& n = w.nodes[ni];
+        const BoxNode4& bn = w.box_nodes[ni];
+        const float* blo[3] = {bn.lo_x, bn.lo_y, bn.lo_z};
+        const float* bhi[3] = {bn.hi_x, bn.hi_y, bn.hi_z};
+        const uint32_t lo[3] = {n.q_lo_x, n.q_lo_y, n.q_lo_z}, hi[3] = {n.q_hi_x, n.q_hi_y, n.q_hi_z};
+        for (int i = 0; i < 4; i++) {
+            if (n.child[i] == kEmptyChild) continue;
+            for (int rep = 0; rep < 4; rep++) {
+                float tgt[3];
+                const int free_axis = int(rng() % 3);
+                for (int k = 0; k < 3; k++)
+                    tgt[k] = k == free_axis ? blo[k][i] + U(rng) * (bhi[k][i] - blo[k][i]) : ((rng() & 1) ? bhi[k][i] : blo[k][i]);
+                const float o[3] = {-0.5f + 2.f * U(rng), -0.5f + 2.f * U(rng), -0.5f + 2.f * U(rng)};
+                float d[3] = {tgt[0] - o[0], tgt[1] - o[1], tgt[2] - o[2]};
+                const float len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (!(len > 0.f)) continue;
+                for (float& x : d) x /= len;
+                const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
+                float tn = 0.f, tf = 3.4e38f;
+                for (int k = 0; k < 3; k++) {
+                    float sc;
+                    const uint32_t bits = ((n.exps >> (8 * k)) & 0xffu) << 23;
+                    std::memcpy(&sc, &bits, 4);
+                    const float a = sc * inv[k], b = (n.origin[k] - o[k]) * inv[k];
+                    const float t0 = std::fma(float((lo[k] >> (8 * i)) & 0xffu), a, b);
+                    const float t1 = std::fma(float((hi[k] >> (8 * i)) & 0xffu), a, b);
+                    tn = std::max(tn, std::min(t0, t1));
+                    tf = std::min(tf, std::max(t0, t1));
+                }
+                grazing++;
+                CHECK(tn <= tf * kBoxRobust, "grazing ray at node %zu child %d rejected: [%g, %g]\n", ni, i, double(tn),
+                      double(tf));
+            }
+        }
+    }
+    std::printf("grazing rays %llu\n", (unsigned long long)grazing);
+    std::printf("nodes %zu, leaves %u, depth %u, box tests %llu (%llu entering), failures %d\n", w.nodes.size(),
+                w.num_leaves, w.max_depth, (unsigned long long)tested, (unsigned long long)entered, failures);
+    std::printf(failures ? "bvh_check: FAILED\n" : "bvh_check: OK\n");
+    return failures ? 1 : 0;
+}

@@ -255,6 +255,7 @@ static int cmd_registry(const std::string& dir) {
     EXPECT(ComponentFactory::CheckRegistered("renderer", "pt.hip"), "renderer pt.hip");
     EXPECT(ComponentFactory::CheckRegistered("renderer", "pt.direct.hip"), "renderer pt.direct.hip");
     EXPECT(ComponentFactory::CheckRegistered("renderer", "pt.mis.hip"), "renderer pt.mis.hip");
+    EXPECT(ComponentFactory::CheckRegistered("renderer", "raycast.hip"), "renderer raycast.hip");
     EXPECT(ComponentFactory::CheckRegistered("rendersched", "hip"), "rendersched hip");
     EXPECT(!ComponentFactory::CheckRegistered("scene", "qbvh"), "qbvh is not in the plugin");
     std::unique_ptr<Scene> scene(ComponentFactory::Create<Scene>("hip.wbvh"));

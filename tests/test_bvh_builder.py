@@ -1,0 +1,20 @@
+"""Host-side invariants of the wide-BVH builder and its 8-bit quantized nodes (no GPU):
+every triangle in exactly one leaf, quantized child boxes containing the exact ones, and
+the device slab test (restated in float with the kernel's operations) accepting every
+ray that enters or grazes an exact child box.  Runs tests/native/bvh_check."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+CHECK = Path(__file__).resolve().parent / "native" / "bvh_check"
+
+
+@pytest.mark.parametrize("tris,seed,leaf", [(20000, 1, 4), (5000, 7, 16), (3000, 3, 1), (1, 5, 4), (2, 9, 4)])
+def test_bvh_invariants(tris, seed, leaf):
+    if not CHECK.exists():
+        pytest.fail("tests/native/bvh_check is not built (run __graft_entry__.build())")
+    p = subprocess.run([str(CHECK), str(tris), str(seed), str(leaf)], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=300)
+    print(p.stdout)
+    assert p.returncode == 0 and "bvh_check: OK" in p.stdout

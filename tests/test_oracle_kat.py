@@ -307,3 +307,32 @@ def test_directly_visible_light_equals_Le(oracle):
         film, _ = o.render("pt", num_samples=16 * 2000)
         # raster sampling is uniform over the film, so pixel means carry multinomial noise of ~1/sqrt(2000)
         np.testing.assert_allclose(film.mean(axis=(0, 1)), (3, 2, 1), rtol=1e-9 if fp64 else 1e-5)
+
+
+def facing_plane_scene(res_w=32, res_h=24, fovy=60.0):
+    """A 4 x 4 quad in the plane z = 0 (normal +z) seen by a camera at (0, 0, 1) looking down -z."""
+    b = scenes._Builder()
+    b.quad((-2, -2, 0), (4, 0, 0), (0, 4, 0), 0)
+    P, N, F, M, L = b.arrays()
+    mats = scenes._materials([(scenes.BSDF_DIFFUSE, (0.5, 0.5, 0.5))])
+    view = scenes.look_at((0, 0, 1), (0, 0, 0), (0, 1, 0)).astype(np.float32)
+    cam = scenes.Camera(view=view, fovy=fovy, width=res_w, height=res_h)
+    rc = scenes.RenderConfig(num_samples=res_w * res_h, max_path_vertices=2, rr_depth=-1, seed=1)
+    return scenes.Scene("plane", P, N, F, M, L, mats, scenes._lights([]), cam, rc)
+
+
+def test_raycast_closed_form(oracle):
+    """raycast.cpp:96-137 on a plane facing the camera: the pixel value |n . -d| is the cosine
+    between the pixel-centre ray and the axis, 1 / sqrt(1 + x^2 + y^2) with
+    x = (2u - 1) tan(fovy / 2) W / H, y = (2v - 1) tan(fovy / 2), (u, v) the pixel centre."""
+    W, H, fovy = 32, 24, 60.0
+    sc = facing_plane_scene(W, H, fovy)
+    img = oracle.Oracle(sc, fp64=True, accel="bvh").raycast()
+    t = np.tan(np.radians(fovy) / 2)
+    u = (0.5 + np.arange(W)) / W
+    v = (0.5 + np.arange(H)) / H
+    x = (2 * u[None, :] - 1) * t * W / H
+    y = (2 * v[:, None] - 1) * t
+    want = 1.0 / np.sqrt(1 + x ** 2 + y ** 2)
+    np.testing.assert_allclose(img[..., 0], want, rtol=1e-6)
+    assert (img[..., 0] == img[..., 1]).all() and (img[..., 1] == img[..., 2]).all()

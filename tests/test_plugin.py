@@ -93,8 +93,9 @@ def test_plugin_render_matches_oracle(oracle, tmp_path, renderer):
 
 @pytest.mark.gpu
 def test_plugin_render_env_light_first(oracle, tmp_path):
-    """An environment light placed first in the light list (as the reference's scene loader
-    puts it) reaches the GPU in the same order: light selection in pt.mis matches the oracle."""
+    """An environment light at the head of the light list reaches the GPU at the same
+    position (the plugin keeps the Primitives' light order): light selection in pt.mis
+    matches the oracle."""
     sc = _with_env_light(scenes.room_scene(32, 32, 16))
     scene_file, out = tmp_path / "room_env.bin", tmp_path / "film.f32"
     write_scene_file(sc, scene_file)
