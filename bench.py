@@ -33,7 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "M path-samples/s + Mrays/s on 1 MI355X at fixed RMSE; backward/forward ratio"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 56   # bytes of a 4-wide node the kernel loads (origin, quanta, 8-bit planes, 4 child words)
+NODE_BYTES = 112  # bytes of a 4-wide node the kernel loads (6 box planes x 4 + 4 child words)
 TRI_BYTES = 48    # one triangle record
 RAY_BYTES = 32    # o, d, tmin, tmax
 HIT_BYTES = 16    # t, b0, b1, prim

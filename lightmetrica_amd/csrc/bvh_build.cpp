@@ -191,15 +191,14 @@ class Collapser {
 
     bool run(uint32_t root, std::string& err) {
         out_.nodes.clear();
-        out_.box_nodes.clear();
         out_.tris.clear();
         out_.tris.reserve(prims_.size());
         out_.num_leaves = 0;
         out_.max_depth = 0;
         if (bn_[root].leaf()) {
             // A single leaf: wrap it in one node so traversal always starts at an inner node.
-            out_.box_nodes.emplace_back();
-            init_node(out_.box_nodes.back());
+            out_.nodes.emplace_back();
+            init_node(out_.nodes.back());
             set_child(0, 0, root, err);
             out_.max_depth = 1;
             return err.empty();
@@ -209,11 +208,12 @@ class Collapser {
     }
 
   private:
-    static void init_node(BoxNode4& n) {
+    static void init_node(Node4& n) {
         for (int i = 0; i < 4; i++) {
             n.lo_x[i] = n.lo_y[i] = n.lo_z[i] = std::numeric_limits<float>::infinity();
             n.hi_x[i] = n.hi_y[i] = n.hi_z[i] = -std::numeric_limits<float>::infinity();
             n.child[i] = kEmptyChild;
+            n.pad[i] = 0;
         }
     }
 
@@ -240,11 +240,11 @@ class Collapser {
 
     void set_child(uint32_t node, int slot, uint32_t b, std::string& err) {
         const Box& bx = bn_[b].box;
-        BoxNode4& n = out_.box_nodes[node];
+        Node4& n = out_.nodes[node];
         n.lo_x[slot] = bx.lo[0]; n.hi_x[slot] = bx.hi[0];
         n.lo_y[slot] = bx.lo[1]; n.hi_y[slot] = bx.hi[1];
         n.lo_z[slot] = bx.lo[2]; n.hi_z[slot] = bx.hi[2];
-        if (bn_[b].leaf()) out_.box_nodes[node].child[slot] = emit_leaf(b, err);
+        if (bn_[b].leaf()) out_.nodes[node].child[slot] = emit_leaf(b, err);
     }
 
     // Greedy collapse: open the largest-area inner child until 4 children.
@@ -265,9 +265,9 @@ class Collapser {
             kids[best] = uint32_t(bn_[open].left);
             kids[nk++] = uint32_t(bn_[open].right);
         }
-        uint32_t me = uint32_t(out_.box_nodes.size());
-        out_.box_nodes.emplace_back();
-        init_node(out_.box_nodes[me]);
+        uint32_t me = uint32_t(out_.nodes.size());
+        out_.nodes.emplace_back();
+        init_node(out_.nodes[me]);
         for (int i = 0; i < nk; i++) {
             set_child(me, i, kids[i], err);
             if (!err.empty()) return me;
@@ -276,7 +276,7 @@ class Collapser {
             if (bn_[kids[i]].leaf()) continue;
             uint32_t c = emit(kids[i], depth + 1, err);
             if (!err.empty()) return me;
-            out_.box_nodes[me].child[i] = c;
+            out_.nodes[me].child[i] = c;
         }
         return me;
     }
@@ -288,54 +288,10 @@ class Collapser {
     WideBVH& out_;
 };
 
-// 8-bit quantization of one node's child boxes (see Node4).  Per axis the quantum is the
-// smallest power of two q with 253 q >= the node's extent; the grid origin lies one quantum
-// below the node box, so planes rounded outward and widened by one quantum stay in 0..255.
-Node4 quantize(const BoxNode4& b) {
-    Node4 n;
-    std::memset(&n, 0, sizeof(n));
-    const float* lo[3] = {b.lo_x, b.lo_y, b.lo_z};
-    const float* hi[3] = {b.hi_x, b.hi_y, b.hi_z};
-    uint32_t* qlo[3] = {&n.q_lo_x, &n.q_lo_y, &n.q_lo_z};
-    uint32_t* qhi[3] = {&n.q_hi_x, &n.q_hi_y, &n.q_hi_z};
-    for (int a = 0; a < 3; a++) {
-        double nlo = INFINITY, nhi = -INFINITY;
-        for (int i = 0; i < 4; i++) {
-            if (b.child[i] == kEmptyChild) continue;
-            nlo = std::min(nlo, double(lo[a][i]));
-            nhi = std::max(nhi, double(hi[a][i]));
-        }
-        if (!(nhi >= nlo)) nlo = nhi = 0.0;  // no children
-        int e = -126;  // the smallest normal quantum the biased exponent can express
-        const double ext = nhi - nlo;
-        if (ext > 0.0) e = std::max(e, int(std::ceil(std::log2(ext / 253.0))));
-        double q = std::ldexp(1.0, e);
-        while (253.0 * q < ext) q *= 2.0, e++;
-        // The origin: one quantum below the box, rounded down to a float.
-        float org = float(nlo - q);
-        if (double(org) > nlo - q) org = std::nextafter(org, -INFINITY);
-        n.origin[a] = org;
-        n.exps |= uint32_t(e + 127) << (8 * a);
-        for (int i = 0; i < 4; i++) {
-            uint32_t ql = 255u, qh = 0u;  // empty slot: an inverted box
-            if (b.child[i] != kEmptyChild) {
-                const double l = std::floor((double(lo[a][i]) - double(org)) / q) - 1.0;
-                const double h = std::ceil((double(hi[a][i]) - double(org)) / q) + 1.0;
-                ql = uint32_t(std::max(0.0, std::min(255.0, l)));
-                qh = uint32_t(std::max(0.0, std::min(255.0, h)));
-            }
-            *qlo[a] |= ql << (8 * i);
-            *qhi[a] |= qh << (8 * i);
-        }
-    }
-    for (int i = 0; i < 4; i++) n.child[i] = b.child[i];
-    return n;
-}
-
 double sah_of(const WideBVH& w, float ct) {
     // Expected cost: sum over nodes of area(node)/area(root) * (ct + ...): computed from child boxes.
-    if (w.box_nodes.empty()) return 0.0;
-    const BoxNode4& r = w.box_nodes[0];
+    if (w.nodes.empty()) return 0.0;
+    const Node4& r = w.nodes[0];
     double root_area = 0.0;
     {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -350,7 +306,7 @@ double sah_of(const WideBVH& w, float ct) {
     }
     if (!(root_area > 0)) return 0.0;
     double cost = ct;  // the root is always visited
-    for (const BoxNode4& n : w.box_nodes) {
+    for (const Node4& n : w.nodes) {
         for (int i = 0; i < 4; i++) {
             if (n.child[i] == kEmptyChild) continue;
             double dx = n.hi_x[i] - n.lo_x[i], dy = n.hi_y[i] - n.lo_y[i], dz = n.hi_z[i] - n.lo_z[i];
@@ -398,8 +354,6 @@ bool build_wide_bvh(const float* positions, uint32_t num_vertices, const uint32_
     Collapser c(b.nodes(), prims, positions, faces, out);
     if (!c.run(root, err)) return false;
     out.sah_cost = sah_of(out, p.traversal_cost);
-    out.nodes.resize(out.box_nodes.size());
-    for (size_t i = 0; i < out.box_nodes.size(); i++) out.nodes[i] = quantize(out.box_nodes[i]);
     out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return true;
 }

@@ -2,8 +2,8 @@
 //
 // Replaces QBVHScene::IntersectTriangles (src/liblightmetrica/scene.qbvh.cpp:837-960):
 //   * one ray per lane, 64-lane wavefronts;
-//   * the 64-byte node (origin, quantum exponents, 8-bit box planes of 4 children, 4 child
-//     words) is fetched with three 16-byte loads and one 8-byte load;
+//   * the node (6 SoA box planes x 4 children + 4 child words) is fetched as seven
+//     16-byte loads from one 128-byte line;
 //   * children hit by the ray are sorted by entry distance; the nearest is descended
 //     into and the others are pushed (closest hit), or all are pushed (any hit);
 //   * the traversal stack lives in LDS, [level][lane] so the 64 lanes of a wave hit
@@ -143,30 +143,23 @@ constexpr uint32_t kDone = 0xffffffffu;
 
 // Slab tests of the four children of inner node `cur`.  Returns how many the ray enters
 // within [tmin, tmax]; with kSort the hit children come first in (key, val), nearest first.
-// A plane at origin + q 2^e gives t = q (2^e / d) + (origin - o) / d: one FMA per plane on
-// the byte converted to float; 2^e / d is exact.  The quanta's one-quantum margin absorbs
-// the rounding of (origin - o) / d for rays closer to the node than 2^23 quanta.
 template <bool kSort>
 __device__ __forceinline__ int node_children(const Node4* __restrict__ nodes, uint32_t cur, const RayIn& r, float ix,
                                              float iy, float iz, float tmin, float tmax, float key[4],
                                              uint32_t val[4]) {
-    const uint4* np = reinterpret_cast<const uint4*>(nodes + cur);
-    const uint4 w0 = np[0], w1 = np[1], w2 = np[2];
-    const uint2 w3 = reinterpret_cast<const uint2*>(np + 3)[0];
-    const float ax = __uint_as_float((w0.w & 0xffu) << 23) * ix;
-    const float ay = __uint_as_float(((w0.w >> 8) & 0xffu) << 23) * iy;
-    const float az = __uint_as_float(((w0.w >> 16) & 0xffu) << 23) * iz;
-    const float bx = (__uint_as_float(w0.x) - r.ox) * ix;
-    const float by = (__uint_as_float(w0.y) - r.oy) * iy;
-    const float bz = (__uint_as_float(w0.z) - r.oz) * iz;
-    const uint32_t c[4] = {w2.z, w2.w, w3.x, w3.y};
+    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+    const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+    const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
     int nhit = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const int sh = 8 * i;
-        const float t0x = fmaf(float((w1.x >> sh) & 0xffu), ax, bx), t1x = fmaf(float((w1.y >> sh) & 0xffu), ax, bx);
-        const float t0y = fmaf(float((w1.z >> sh) & 0xffu), ay, by), t1y = fmaf(float((w1.w >> sh) & 0xffu), ay, by);
-        const float t0z = fmaf(float((w2.x >> sh) & 0xffu), az, bz), t1z = fmaf(float((w2.y >> sh) & 0xffu), az, bz);
+        const float t0x = (lx[i] - r.ox) * ix, t1x = (hx[i] - r.ox) * ix;
+        const float t0y = (ly[i] - r.oy) * iy, t1y = (hy[i] - r.oy) * iy;
+        const float t0z = (lz[i] - r.oz) * iz, t1z = (hz[i] - r.oz) * iz;
         const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
         const bool hit = c[i] != kEmptyChild && tn <= tf * kBoxRobust;
@@ -293,7 +286,7 @@ __device__ __forceinline__ bool traverse(const Node4* __restrict__ nodes, const 
 // Lanes of a persistent traversal fetch new rays once this many lanes of the wave are idle
 // (a build-time tuning knob).
 #ifndef LMGPU_REFILL_IDLE
-#define LMGPU_REFILL_IDLE 16
+#define LMGPU_REFILL_IDLE 40
 #endif
 constexpr int kRefillIdle = LMGPU_REFILL_IDLE;
 

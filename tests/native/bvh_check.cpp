@@ -5,12 +5,12 @@
 //
 // Checks, on a random triangle soup (uniform centroids, edges up to 0.05):
 //   1. every triangle is referenced by exactly one leaf record;
-//   2. every quantized child box, dequantized exactly, contains the exact child box and
-//      every triangle below that child;
-//   3. the device slab test restated in float (same operations as traverse.hpp) never
-//      rejects a child whose exact box a random ray enters, for rays from inside and
-//      around the scene (conservativeness of the quantized planes and the robust test);
-//   4. nor a ray aimed at a corner or an edge point of the exact child box (grazing rays).
+//   2. every child box contains every triangle below that child;
+//   3. the device slab test restated in float (the operations of traverse.hpp: reciprocal
+//      direction, (plane - o) * inv, exit distance scaled by 1 + 2 gamma(3)) never rejects
+//      a child whose box a random ray enters (exact arithmetic), for rays from inside and
+//      around the scene;
+//   4. nor a ray aimed at a corner or an edge point of the child box (grazing rays).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,12 +31,6 @@ static int failures = 0;
             failures++;                                 \
         }                                               \
     } while (0)
-
-static double plane_exact(const Node4& n, int axis, uint32_t word, int i) {
-    const uint32_t q = (word >> (8 * i)) & 0xffu;
-    const int e = int((n.exps >> (8 * axis)) & 0xffu) - 127;
-    return double(n.origin[axis]) + std::ldexp(double(q), e);
-}
 
 struct Box {
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -65,19 +59,15 @@ static Box subtree_box(const WideBVH& w, uint32_t ref, std::vector<int>& seen) {
         return b;
     }
     const Node4& n = w.nodes[ref];
-    const uint32_t lo[3] = {n.q_lo_x, n.q_lo_y, n.q_lo_z}, hi[3] = {n.q_hi_x, n.q_hi_y, n.q_hi_z};
+    const float* lo[3] = {n.lo_x, n.lo_y, n.lo_z};
+    const float* hi[3] = {n.hi_x, n.hi_y, n.hi_z};
     for (int i = 0; i < 4; i++) {
         if (n.child[i] == kEmptyChild) continue;
         Box c = subtree_box(w, n.child[i], seen);
-        const BoxNode4& bn = w.box_nodes[ref];
-        const float* blo[3] = {bn.lo_x, bn.lo_y, bn.lo_z};
-        const float* bhi[3] = {bn.hi_x, bn.hi_y, bn.hi_z};
         for (int a = 0; a < 3; a++) {
-            CHECK(plane_exact(n, a, lo[a], i) <= c.lo[a] && plane_exact(n, a, hi[a], i) >= c.hi[a],
-                  "node %u child %d axis %d: quantized [%g, %g] does not contain [%g, %g]\n", ref, i, a,
-                  plane_exact(n, a, lo[a], i), plane_exact(n, a, hi[a], i), c.lo[a], c.hi[a]);
-            CHECK(plane_exact(n, a, lo[a], i) <= blo[a][i] && plane_exact(n, a, hi[a], i) >= bhi[a][i],
-                  "node %u child %d axis %d: quantized box does not contain the exact child box\n", ref, i, a);
+            CHECK(double(lo[a][i]) <= c.lo[a] && double(hi[a][i]) >= c.hi[a],
+                  "node %u child %d axis %d: box [%g, %g] does not contain [%g, %g]\n", ref, i, a, double(lo[a][i]),
+                  double(hi[a][i]), c.lo[a], c.hi[a]);
             b.lo[a] = std::min(b.lo[a], c.lo[a]);
             b.hi[a] = std::max(b.hi[a], c.hi[a]);
         }
@@ -123,28 +113,17 @@ int main(int argc, char** argv) {
         const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
         for (size_t ni = 0; ni < w.nodes.size(); ni += 7) {
             const Node4& n = w.nodes[ni];
-            const BoxNode4& bn = w.box_nodes[ni];
-            float a[3], b[3];
-            for (int k = 0; k < 3; k++) {
-                float sc;
-                const uint32_t bits = ((n.exps >> (8 * k)) & 0xffu) << 23;
-                std::memcpy(&sc, &bits, 4);
-                a[k] = sc * inv[k];
-                b[k] = (n.origin[k] - o[k]) * inv[k];
-            }
-            const uint32_t lo[3] = {n.q_lo_x, n.q_lo_y, n.q_lo_z}, hi[3] = {n.q_hi_x, n.q_hi_y, n.q_hi_z};
-            const float* blo[3] = {bn.lo_x, bn.lo_y, bn.lo_z};
-            const float* bhi[3] = {bn.hi_x, bn.hi_y, bn.hi_z};
+            const float* blo[3] = {n.lo_x, n.lo_y, n.lo_z};
+            const float* bhi[3] = {n.hi_x, n.hi_y, n.hi_z};
             for (int i = 0; i < 4; i++) {
                 if (n.child[i] == kEmptyChild) continue;
                 float tn = 0.f, tf = 3.4e38f;
                 double etn = 0.0, etf = INFINITY;
                 for (int k = 0; k < 3; k++) {
-                    const float t0 = std::fma(float((lo[k] >> (8 * i)) & 0xffu), a[k], b[k]);
-                    const float t1 = std::fma(float((hi[k] >> (8 * i)) & 0xffu), a[k], b[k]);
+                    const float t0 = (blo[k][i] - o[k]) * inv[k], t1 = (bhi[k][i] - o[k]) * inv[k];
                     tn = std::max(tn, std::min(t0, t1));
                     tf = std::min(tf, std::max(t0, t1));
-                    // exact entry / exit of the unquantized child box
+                    // exact entry / exit of the child box
                     const double dd = double(d[k]);
                     if (dd == 0.0) {
                         if (o[k] < blo[k][i] || o[k] > bhi[k][i]) etf = -1.0;
@@ -168,10 +147,8 @@ int main(int argc, char** argv) {
     uint64_t grazing = 0;
     for (size_t ni = 0; ni < w.nodes.size(); ni++) {
         const Node4& n = w.nodes[ni];
-        const BoxNode4& bn = w.box_nodes[ni];
-        const float* blo[3] = {bn.lo_x, bn.lo_y, bn.lo_z};
-        const float* bhi[3] = {bn.hi_x, bn.hi_y, bn.hi_z};
-        const uint32_t lo[3] = {n.q_lo_x, n.q_lo_y, n.q_lo_z}, hi[3] = {n.q_hi_x, n.q_hi_y, n.q_hi_z};
+        const float* blo[3] = {n.lo_x, n.lo_y, n.lo_z};
+        const float* bhi[3] = {n.hi_x, n.hi_y, n.hi_z};
         for (int i = 0; i < 4; i++) {
             if (n.child[i] == kEmptyChild) continue;
             for (int rep = 0; rep < 4; rep++) {
@@ -187,12 +164,7 @@ int main(int argc, char** argv) {
                 const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
                 float tn = 0.f, tf = 3.4e38f;
                 for (int k = 0; k < 3; k++) {
-                    float sc;
-                    const uint32_t bits = ((n.exps >> (8 * k)) & 0xffu) << 23;
-                    std::memcpy(&sc, &bits, 4);
-                    const float a = sc * inv[k], b = (n.origin[k] - o[k]) * inv[k];
-                    const float t0 = std::fma(float((lo[k] >> (8 * i)) & 0xffu), a, b);
-                    const float t1 = std::fma(float((hi[k] >> (8 * i)) & 0xffu), a, b);
+                    const float t0 = (blo[k][i] - o[k]) * inv[k], t1 = (bhi[k][i] - o[k]) * inv[k];
                     tn = std::max(tn, std::min(t0, t1));
                     tf = std::min(tf, std::max(t0, t1));
                 }

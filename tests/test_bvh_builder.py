@@ -1,7 +1,7 @@
-"""Host-side invariants of the wide-BVH builder and its 8-bit quantized nodes (no GPU):
-every triangle in exactly one leaf, quantized child boxes containing the exact ones, and
-the device slab test (restated in float with the kernel's operations) accepting every
-ray that enters or grazes an exact child box.  Runs tests/native/bvh_check."""
+"""Host-side invariants of the wide-BVH builder (no GPU): every triangle in exactly one
+leaf, child boxes containing their triangles, and the device slab test (restated in float
+with the kernel's operations and robust exit scaling) accepting every ray that enters or
+grazes a child box.  Runs tests/native/bvh_check."""
 import subprocess
 from pathlib import Path
 
