@@ -9,8 +9,9 @@
 //   * the traversal stack lives in LDS, [level][lane] so the 64 lanes of a wave hit
 //     distinct banks, and spills to per-lane private memory beyond kStackLDS levels;
 //   * boxes are tested with the reciprocal-direction slab test and the exit distance
-//     is scaled by 1 + 2*gamma(3) so rounding never culls a box the ray touches
-//     (Ize, "Robust BVH Ray Traversal", JCGT 2013);
+//     is scaled by 1 + 2*gamma(4) so rounding never culls a box the ray touches
+//     (Ize, "Robust BVH Ray Traversal", JCGT 2013; one more gamma term for the 1-ulp
+//     hardware reciprocal);
 //   * triangles use the watertight test of Woop, Benthin and Wald (JCGT 2013), with
 //     the FP64 fallback for edge functions that evaluate to exactly zero.
 // Acceptance follows the reference's QBVH: t strictly inside (tmin, tmax), the first
@@ -63,9 +64,12 @@ __device__ __forceinline__ Shear make_shear(const RayIn& r) {
         s.kx = s.ky;
         s.ky = t;
     }
-    s.Sx = pick3(r.dx, r.dy, r.dz, s.kx) / dkz;
-    s.Sy = pick3(r.dx, r.dy, r.dz, s.ky) / dkz;
-    s.Sz = 1.f / dkz;
+    // The shear only has to be the same for every triangle of the ray, so the hardware
+    // reciprocal (1 ulp) serves.
+    const float inv = __builtin_amdgcn_rcpf(dkz);
+    s.Sx = pick3(r.dx, r.dy, r.dz, s.kx) * inv;
+    s.Sy = pick3(r.dx, r.dy, r.dz, s.ky) * inv;
+    s.Sz = inv;
     s.ox = pick3(r.ox, r.oy, r.oz, s.kx);
     s.oy = pick3(r.ox, r.oy, r.oz, s.ky);
     s.oz = pick3(r.ox, r.oy, r.oz, s.kz);
@@ -102,7 +106,7 @@ __device__ __forceinline__ bool watertight(const Shear& s, float4 v0, float4 v1,
     det = U + V + W;
     if (det == 0.f) return false;
     const float T = U * (s.Sz * Az) + V * (s.Sz * Bz) + W * (s.Sz * Cz);
-    t = T / det;
+    t = T * __builtin_amdgcn_rcpf(det);  // within 3 ulp of T / det
     return t > tmin && t < tmax;
 }
 
@@ -127,7 +131,9 @@ __device__ __forceinline__ void refine_fp64(const RayIn& r, float4 v0, float4 v1
     b1 = float(v);
 }
 
-constexpr float kBoxRobust = 1.0f + 2.0f * 1.8e-7f;  // 1 + 2*gamma(3), gamma(n) = n*u/(1-n*u), u = 2^-24
+// 1 + 2 gamma(4), gamma(n) = n u / (1 - n u), u = 2^-24: (plane - o) and * inv round once each
+// and the hardware reciprocal is within 1 ulp (2u).
+constexpr float kBoxRobust = 1.0f + 2.0f * 2.4e-7f;
 
 __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
     const bool s = kb < ka;
@@ -202,7 +208,10 @@ __device__ __forceinline__ bool leaf_hits(const TriRecord* __restrict__ tris, ui
     return any;
 }
 
-__device__ __forceinline__ float safe_rcp(float d) { return 1.f / (fabsf(d) > 1e-30f ? d : copysignf(1e-30f, d)); }
+// Hardware reciprocal (v_rcp_f32, within 1 ulp) of a direction component kept away from 0.
+__device__ __forceinline__ float safe_rcp(float d) {
+    return __builtin_amdgcn_rcpf(fabsf(d) > 1e-30f ? d : copysignf(1e-30f, d));
+}
 
 // The per-lane traversal stack: levels [0, kStackLDS) live in LDS (`lds` is this lane's
 // column of the block's __shared__ array, lds[level * kTraceBlock]), deeper levels in

@@ -6,8 +6,8 @@
 // Checks, on a random triangle soup (uniform centroids, edges up to 0.05):
 //   1. every triangle is referenced by exactly one leaf record;
 //   2. every child box contains every triangle below that child;
-//   3. the device slab test restated in float (the operations of traverse.hpp: reciprocal
-//      direction, (plane - o) * inv, exit distance scaled by 1 + 2 gamma(3)) never rejects
+//   3. the device slab test restated in float (the operations of traverse.hpp: 1-ulp
+//      reciprocal direction, (plane - o) * inv, exit distance scaled by 1 + 2 gamma(4)) never rejects
 //      a child whose box a random ray enters (exact arithmetic), for rays from inside and
 //      around the scene;
 //   4. nor a ray aimed at a corner or an edge point of the child box (grazing rays).
@@ -75,7 +75,14 @@ static Box subtree_box(const WideBVH& w, uint32_t ref, std::vector<int>& seen) {
     return b;
 }
 
-static float safe_rcp(float d) { return 1.f / (std::fabs(d) > 1e-30f ? d : std::copysign(1e-30f, d)); }
+// The kernel's reciprocal is the hardware one (within 1 ulp): emulate the worst case by
+// moving the correctly rounded reciprocal one ulp up or down at random.
+static std::mt19937 ulp_rng(12345);
+static float safe_rcp(float d) {
+    const float r = 1.f / (std::fabs(d) > 1e-30f ? d : std::copysign(1e-30f, d));
+    const uint32_t pick = ulp_rng() % 3;
+    return pick == 0 ? r : std::nextafter(r, pick == 1 ? INFINITY : -INFINITY);
+}
 
 int main(int argc, char** argv) {
     const uint32_t nt = argc > 1 ? uint32_t(std::atoi(argv[1])) : 20000;
@@ -104,7 +111,7 @@ int main(int argc, char** argv) {
     for (uint32_t t = 0; t < nt; t++) CHECK(seen[t] == 1, "triangle %u referenced %d times\n", t, seen[t]);
 
     // 3. Ray / child-box tests: float restatement of node_children against the exact boxes.
-    const float kBoxRobust = 1.0f + 2.0f * 1.8e-7f;
+    const float kBoxRobust = 1.0f + 2.0f * 2.4e-7f;
     uint64_t tested = 0, entered = 0;
     for (int r = 0; r < 20000; r++) {
         const float o[3] = {-0.5f + 2.f * U(rng), -0.5f + 2.f * U(rng), -0.5f + 2.f * U(rng)};

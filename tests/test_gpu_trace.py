@@ -175,10 +175,16 @@ def test_full_size_properties(gpu, oracle):
     gpu.upload(sc)
     info = gpu.build_bvh()
     print(f"BVH: {info}", flush=True)
+    t32, c0, c1, p32 = gpu.trace_closest(rays, refine_fp64=False)
+    h32 = p32 >= 0
+    # the watertight test's own barycentrics lie inside the triangle (same-sign edge functions)
+    assert np.all(c0[h32] >= 0) and np.all(c1[h32] >= 0) and np.all(c0[h32] + c1[h32] <= 1 + 1e-6)
     t, b0, b1, prim = gpu.trace_closest(rays, refine_fp64=True)
     hit = prim >= 0
-    # barycentrics inside the triangle, t inside (tmin, tmax)
-    assert np.all(b0[hit] >= -1e-6) and np.all(b1[hit] >= -1e-6) and np.all(b0[hit] + b1[hit] <= 1 + 1e-6)
+    np.testing.assert_array_equal(prim, p32)
+    # FP64-refined barycentrics of a ray the FP32 test accepted on an edge may lie just outside:
+    # the FP32 edge functions resolve ~1e-7 of the coordinates (~1), i.e. ~1e-5 of a 0.01 triangle
+    assert np.all(b0[hit] >= -1e-5) and np.all(b1[hit] >= -1e-5) and np.all(b0[hit] + b1[hit] <= 1 + 1e-5)
     assert np.all(t[hit] > 0)
     # any hit agrees with closest hit on every ray
     occ = gpu.trace_any(rays)
