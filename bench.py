@@ -82,7 +82,7 @@ def run_traversal(args, torch, Device, scenes):
     log(f"[bench] generated {sc.num_triangles} triangles and {len(rays)} rays in {time.time() - t0:.1f}s")
     dev = Device(0)
     dev.upload(sc)
-    info = dev.build_bvh(max_leaf_size=args.leaf)
+    info = dev.build_bvh(max_leaf_size=args.leaf, sah_bins=args.sah_bins, traversal_cost=args.traversal_cost)
     log(f"[bench] BVH: {info.num_nodes} nodes, depth {info.max_depth}, {info.build_seconds:.2f}s, "
         f"{info.device_bytes / 2**20:.1f} MiB")
     cols = [torch.from_numpy(rays.o[:, k].copy()) for k in range(3)] + \
@@ -341,6 +341,32 @@ def cpu_baseline_pt(sc, samples):
                       f"std::mt19937 streams), {threads} threads of {cpu_info()}"}
 
 
+def equal_rmse_check(sc, Device, samples=1 << 20):
+    """The room at equal sample counts: the GPU image against the FP64 oracle drawing the same
+    Philox streams (arithmetic difference), next to the Monte-Carlo RMSE of one image (from two
+    independent GPU renders).  With the estimator identical on both sides, equal sample counts
+    give equal RMSE when the first is far below the second."""
+    import numpy as np
+
+    from oracle import lmref
+
+    dev = Device(0)
+    dev.upload(sc)
+    dev.build_bvh()
+    g1, _ = dev.render("pt", num_samples=samples, seed=1, film_precision="fp64")
+    g2, _ = dev.render("pt", num_samples=samples, seed=2, film_precision="fp64")
+    dev.close()
+    o1, _ = lmref.Oracle(sc, fp64=True, accel="bvh").render("pt", num_samples=samples, seed=1, threads=host_threads())
+    c1, _ = lmref.Oracle(sc, fp64=False, accel="qbvh", approx_rsqrt=True).render(
+        "pt", num_samples=samples, seed=1, threads=host_threads())
+    mc = rmse(g1, g2) / np.sqrt(2.0)
+    return {"samples": samples,
+            "rmse_gpu_vs_fp64_oracle_same_streams": rmse(g1, o1),
+            "rmse_cpu_fp32_vs_fp64_oracle_same_streams": rmse(c1, o1),
+            "monte_carlo_rmse_per_image": mc,
+            "gpu_deviation_over_mc_rmse": rmse(g1, o1) / mc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -349,6 +375,8 @@ def main():
     ap.add_argument("--triangles", type=int, default=1_000_000)
     ap.add_argument("--rays", type=int, default=16_000_000)
     ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--sah-bins", type=int, default=16)
+    ap.add_argument("--traversal-cost", type=float, default=1.2)
     ap.add_argument("--pt-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pt", action="store_true")
@@ -444,6 +472,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline_traversal(sc, rays, args.cpu_sample_rays)
         if not args.no_pt:
             line["path_tracing"]["cpu_baseline"] = cpu_baseline_pt(room, args.cpu_sample_paths)
+            line["path_tracing"]["equal_rmse"] = equal_rmse_check(room, Device)
             line["path_tracing"]["speedup_vs_cpu"] = round(
                 line["path_tracing"]["M_path_samples_per_s"] / line["path_tracing"]["cpu_baseline"]["value"], 1)
         line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)

@@ -87,7 +87,7 @@ def build_plugin() -> Path:
 def build_driver() -> Path:
     """The C++ host that loads plugins from a directory like the reference's tests do."""
     tmp = DRIVER.with_name(DRIVER.name + ".tmp")
-    _run(["g++", *HOST_FLAGS, str(ROOT / "tests" / "native" / "plugin_driver.cpp"), "-o", str(tmp), "-ldl"])
+    _run(["g++", *HOST_FLAGS, str(ROOT / "tests" / "native" / "plugin_driver.cpp"), "-o", str(tmp), "-ldl", "-lpthread"])
     os.replace(tmp, DRIVER)
     return DRIVER
 

@@ -23,6 +23,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -290,8 +291,12 @@ class HIPScene final : public Scene {
     }
 
     // Scene::IntersectTriangles: one closest-hit query on the GPU (FP64-refined winner).
+    // The reference calls it from every render thread at once (sched.mt.cpp:139-165); one
+    // context is driven by one thread at a time, so the queries are serialised.  Batches of
+    // rays belong on lmgpu_trace_closest.
     bool IntersectTriangles(Ray& ray, Intersection& isect) const override {
         if (!ctx_) return false;
+        std::lock_guard<std::mutex> lock(mutex_);
         float ox = ray.o.x, oy = ray.o.y, oz = ray.o.z, dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
         float tmin = ray.minT, tmax = ray.maxT;
         lmgpu_rays_soa rs{&ox, &oy, &oz, &dx, &dy, &dz, &tmin, &tmax, 0};
@@ -318,6 +323,7 @@ class HIPScene final : public Scene {
     int device_ = 0;
     int leaf_ = 4;
     lmgpu_ctx* ctx_ = nullptr;
+    mutable std::mutex mutex_;
     lmgpu_bvh_info info_{};
     AABB aabb_;
     std::vector<std::pair<uint32_t, uint32_t>> tri_map_;  // flattened triangle -> (primitive, face)
