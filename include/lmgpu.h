@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define LMGPU_ABI_VERSION 3
+#define LMGPU_ABI_VERSION 4
 
 typedef int lmgpu_status;
 #define LMGPU_OK 0
@@ -165,6 +165,10 @@ typedef struct lmgpu_pt_params {
     uint64_t sample_offset;   /* global index of the first sample (shards of a larger render) */
     uint32_t wavefront_width; /* paths in flight per batch; 0 = auto */
     int film_precision;       /* LMGPU_FILM_* accumulation type on the device */
+    /* Optional, renderer LMGPU_PT only: a device buffer of num_samples float4.  The render
+     * writes each sample's radiance (rgb) and film pixel (w, as uint32 bits); a backward with
+     * the same buffer reads them instead of replaying the paths once more.  NULL: unused. */
+    void* sample_record;
 } lmgpu_pt_params;
 
 /* Output film: width*height*3 values, rows bottom-up (raster y = 0 is the bottom row),
@@ -186,7 +190,8 @@ typedef struct lmgpu_adjoint {
     double* d_albedo;       /* out: num_materials*3 */
     double* d_emission;     /* out: num_lights*3 (may be NULL without lights) */
     int grad_device;        /* d_albedo / d_emission are device pointers */
-    double replay_residual; /* out: sum|Q left at path ends| / sum|Q|, ~1e-16 when the replay retraced every path */
+    double replay_residual; /* out: sum|Q left at path ends| / sum|Q|: ~1e-16 when the replay retraced every
+                               path (~1e-7 with a sample record, whose totals were summed in FP32) */
 } lmgpu_adjoint;
 
 typedef struct lmgpu_pt_stats {

@@ -82,7 +82,7 @@ class HitsSoA(C.Structure):
 class PtParams(C.Structure):
     _fields_ = [("renderer", C.c_int), ("num_samples", C.c_uint64), ("max_path_vertices", C.c_int),
                 ("rr_depth", C.c_int), ("seed", C.c_uint64), ("sample_offset", C.c_uint64),
-                ("wavefront_width", C.c_uint32), ("film_precision", C.c_int)]
+                ("wavefront_width", C.c_uint32), ("film_precision", C.c_int), ("sample_record", C.c_void_p)]
 
 
 class Film(C.Structure):

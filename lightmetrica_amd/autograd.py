@@ -9,7 +9,9 @@ context drives.  The forward uploads them (lmgpu_scene_set_params) and renders; 
 backward runs lmgpu_pt_backward with the same sample streams, so the gradient is that of
 the very estimate the forward returned.  GPU tensors are passed as device pointers and
 the context is switched to torch's current stream for the call, so no host copy of the
-image or its gradient is made.
+image or its gradient is made.  For renderer "pt" on the GPU the forward also records
+each sample's radiance (16 bytes per sample, up to RECORD_LIMIT bytes), so the backward
+replays the paths once instead of twice.
 """
 from __future__ import annotations
 
@@ -18,7 +20,9 @@ import torch
 
 from .gpu import Device
 
-__all__ = ["PTRender", "render"]
+__all__ = ["PTRender", "render", "RECORD_LIMIT"]
+
+RECORD_LIMIT = 4 << 30  # bytes of per-sample record the forward may allocate
 
 
 def _params(t: torch.Tensor) -> np.ndarray:
@@ -32,18 +36,22 @@ class PTRender(torch.autograd.Function):
         sc = device.scene
         shape = (sc.camera.height, sc.camera.width, 3)
         on_gpu = albedo.is_cuda
+        record = None
         if on_gpu:
             img = torch.empty(shape, dtype=torch.float32, device=albedo.device)
+            n = int(opts.get("num_samples") or sc.render.num_samples)
+            if renderer == "pt" and 16 * n <= RECORD_LIMIT and (albedo.requires_grad or emission.requires_grad):
+                record = torch.empty((n, 4), dtype=torch.float32, device=albedo.device)
             device.set_stream(torch.cuda.current_stream(albedo.device).cuda_stream)
             try:
                 device.render(renderer, film_precision="fp64", out_precision="fp32", device_out_ptr=img.data_ptr(),
-                              **opts)
+                              sample_record_ptr=None if record is None else record.data_ptr(), **opts)
             finally:
                 device.set_stream(None)
         else:
             out, _ = device.render(renderer, film_precision="fp64", out_precision="fp32", **opts)
             img = torch.from_numpy(out)
-        ctx.device, ctx.renderer, ctx.opts = device, renderer, opts
+        ctx.device, ctx.renderer, ctx.opts, ctx.record = device, renderer, opts, record
         ctx.shapes = (albedo.shape, emission.shape)
         ctx.save_for_backward(albedo, emission)
         return img
@@ -61,7 +69,7 @@ class PTRender(torch.autograd.Function):
             device.set_stream(torch.cuda.current_stream(g.device).cuda_stream)
             try:
                 device.backward(ctx.renderer, g.data_ptr(), d_albedo_ptr=ga.data_ptr(), d_emission_ptr=ge.data_ptr(),
-                                **ctx.opts)
+                                sample_record_ptr=None if ctx.record is None else ctx.record.data_ptr(), **ctx.opts)
             finally:
                 device.set_stream(None)
             ge = ge[: emission.shape[0]]

@@ -70,6 +70,7 @@ struct PTConsts {
     double* qacc;        // adjoint: per batch sample, 4 doubles (Q.xyz)
     double* grad_mat;    // adjoint: num_materials * 3
     double* grad_light;  // adjoint: num_lights * 3
+    float4* record;      // pt render: per batch sample (radiance, pixel bits), or nullptr
     uint64_t seed;
     uint64_t base;  // global index of the batch's first sample
     uint32_t batch;  // samples in this batch
@@ -572,7 +573,9 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
         rng.next();
         float pdf;
         const V3 d = cam_direction(c.cam, r0, r1, pdf);
-        store_path(out, i, cam_p, 0.f, d, pdf, v3s(1.f), v3s(0.f), i, 1u, rng.dim, pixel_of(c.cam, r0, r1));
+        const uint32_t pixel = pixel_of(c.cam, r0, r1);
+        store_path(out, i, cam_p, 0.f, d, pdf, v3s(1.f), v3s(0.f), i, 1u, rng.dim, pixel);
+        if (kMode == kRender && c.record) c.record[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(pixel));
         return;
     }
     const bool mis = c.renderer == LMGPU_PT_MIS;
@@ -714,7 +717,10 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
     }
     const uint32_t slot = block_append<0>(out_count, alive);
     if (alive) store_path(out, slot, s.p, kEps, wo, pdf, T, L, meta.x, uint32_t(n), rng.dim, meta.w);
-    else if (kMode == kRender && valid) film_add(c, meta.w, L);
+    else if (kMode == kRender && valid) {
+        film_add(c, meta.w, L);
+        if (c.record) c.record[meta.x] = make_float4(L.x, L.y, L.z, __uint_as_float(meta.w));
+    }
     };
     if (kMode == kAdjGrad) {
         for (uint32_t tile = blockIdx.x; tile * kPTBlock < count; tile += gridDim.x) tile_body(tile);
@@ -974,6 +980,19 @@ __global__ __launch_bounds__(256) void k_abs_sum(const double* __restrict__ q, u
     if (threadIdx.x == 0) unsafeAtomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
+// Adjoint pass 1 from a pt sample record: Q = dL/dI[pixel] * radiance.
+__global__ void k_q_from_record(const float4* __restrict__ rec, const float* __restrict__ dLdI, uint32_t n,
+                                double* __restrict__ q) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 r = rec[i];
+    const float* g = dLdI + 3ull * __float_as_uint(r.w);
+    q[4ull * i + 0] = double(g[0]) * double(r.x);
+    q[4ull * i + 1] = double(g[1]) * double(r.y);
+    q[4ull * i + 2] = double(g[2]) * double(r.z);
+    q[4ull * i + 3] = 0.0;
+}
+
 __global__ void k_scale_f64(double* p, uint32_t n, double w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] *= w;
@@ -1037,6 +1056,8 @@ static lmgpu_status validate(lmgpu_ctx* ctx, const lmgpu_pt_params* p, const cha
         return fail(ctx, LMGPU_ERR_UNSUPPORTED, f + ": max_path_vertices must be in [1, 64]");
     if (3u * ctx->bvh_max_depth > uint32_t(kStackTotal))
         return fail(ctx, LMGPU_ERR_BVH_TOO_DEEP, f + ": BVH deeper than the traversal stack");
+    if (p->sample_record && p->renderer != LMGPU_PT)
+        return fail(ctx, LMGPU_ERR_UNSUPPORTED, f + ": a sample record needs renderer LMGPU_PT");
     return LMGPU_OK;
 }
 
@@ -1243,8 +1264,10 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
     hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((acc_bytes / 16 + 255) / 256)), dim3(256), 0, s,
                        static_cast<uint4*>(ctx->film_acc), acc_bytes / 16);
     hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, pl.b.counters, (4096u + 64u) / 4u);
-    for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap)
+    for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap) {
+        pl.c.record = p->sample_record ? static_cast<float4*>(p->sample_record) + base : nullptr;
         st = pl.batch<kRender>(p->sample_offset + base, uint32_t(std::min<uint64_t>(pl.cap, p->num_samples - base)));
+    }
     if (st == LMGPU_OK) {
         // Resolve: sums * W*H/N into the output film.
         const dim3 rg(uint32_t((nval + 255) / 256));
@@ -1347,10 +1370,21 @@ lmgpu_status run_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_adj
     hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, pl.b.counters, (4096u + 64u) / 4u);
     for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap) {
         const uint32_t nb = uint32_t(std::min<uint64_t>(pl.cap, p->num_samples - base));
-        hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((size_t(nb) * 2 + 255) / 256)), dim3(256), 0, s,
-                           reinterpret_cast<uint4*>(pl.c.qacc), size_t(nb) * 2);
-        st = pl.batch<kAdjTotal>(p->sample_offset + base, nb);
-        if (st != LMGPU_OK) break;
+        if (p->sample_record) {
+            // The forward's record holds every sample's radiance: Q without a replay.
+            hipLaunchKernelGGL(k_q_from_record, dim3((nb + 255) / 256), dim3(256), 0, s,
+                               static_cast<const float4*>(p->sample_record) + base, pl.c.dLdI, nb, pl.c.qacc);
+            e = hipGetLastError();
+            if (e != hipSuccess) {
+                st = hip_fail(ctx, e, "k_q_from_record");
+                break;
+            }
+        } else {
+            hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((size_t(nb) * 2 + 255) / 256)), dim3(256), 0, s,
+                               reinterpret_cast<uint4*>(pl.c.qacc), size_t(nb) * 2);
+            st = pl.batch<kAdjTotal>(p->sample_offset + base, nb);
+            if (st != LMGPU_OK) break;
+        }
         const dim3 grid_sum(std::min<uint32_t>((nb + 255) / 256, 256u));
         hipLaunchKernelGGL(k_abs_sum, grid_sum, dim3(256), 0, s, pl.c.qacc, nb, sums + 0);
         st = pl.batch<kAdjGrad>(p->sample_offset + base, nb);

@@ -190,10 +190,12 @@ class Device:
     def render(self, renderer: str = "pt", num_samples: Optional[int] = None, max_path_vertices: Optional[int] = None,
                rr_depth: Optional[int] = None, seed: Optional[int] = None, sample_offset: int = 0,
                wavefront_width: int = 0, film_precision: str = "fp32", out_precision: str = "fp64",
-               device_out_ptr: Optional[int] = None):
+               device_out_ptr: Optional[int] = None, sample_record_ptr: Optional[int] = None):
         """Render the uploaded scene.  Returns (film (H, W, 3) or None for device output, RenderStats).
 
         Film rows are bottom-up (raster y = 0 is the bottom row), values rescaled by W*H/N.
+        sample_record_ptr: device buffer of num_samples float4 (renderer "pt" only) that receives
+        each sample's radiance and pixel, for a backward that skips its first replay.
         """
         sc = self.scene
         if sc is None or sc.camera is None:
@@ -208,6 +210,7 @@ class Device:
         p.sample_offset = int(sample_offset)
         p.wavefront_width = int(wavefront_width)
         p.film_precision = FILM_PRECISIONS[film_precision]
+        p.sample_record = None if sample_record_ptr is None else int(sample_record_ptr)
         W, H = sc.camera.width, sc.camera.height
         film = N.Film()
         film.precision = N.LMGPU_FILM_FP64 if out_precision == "fp64" else N.LMGPU_FILM_FP32
@@ -255,8 +258,12 @@ class Device:
     def backward(self, renderer: str, d_image, num_samples: Optional[int] = None,
                  max_path_vertices: Optional[int] = None, rr_depth: Optional[int] = None,
                  seed: Optional[int] = None, sample_offset: int = 0, wavefront_width: int = 0,
-                 d_albedo_ptr: Optional[int] = None, d_emission_ptr: Optional[int] = None):
+                 d_albedo_ptr: Optional[int] = None, d_emission_ptr: Optional[int] = None,
+                 sample_record_ptr: Optional[int] = None):
         """Path-replay adjoint of render(renderer, ...) with the same arguments.
+
+        sample_record_ptr: the record a "pt" render wrote (same arguments); the backward then
+        takes the per-sample totals from it and replays the paths once instead of twice.
 
         d_image: dL/dI, (H, W, 3) float32 numpy array, or an int device pointer to such data.
         Returns (d_albedo (num_materials, 3), d_emission (num_lights, 3), info) -- the arrays
@@ -275,6 +282,7 @@ class Device:
         p.sample_offset = int(sample_offset)
         p.wavefront_width = int(wavefront_width)
         p.film_precision = N.LMGPU_FILM_FP64
+        p.sample_record = None if sample_record_ptr is None else int(sample_record_ptr)
         adj = N.Adjoint()
         keep = []
         if isinstance(d_image, int):

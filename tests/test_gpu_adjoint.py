@@ -160,7 +160,10 @@ def test_linear_loss_gradient_is_the_emission_image(gpu):
         np.testing.assert_allclose(ge[l], want, rtol=1e-4)
 
 
-def test_autograd_function(gpu):
+@pytest.mark.parametrize("renderer", ["pt.mis", "pt"])
+def test_autograd_function(gpu, renderer):
+    """render() as a torch.autograd.Function on CPU and GPU tensors (for "pt" on the GPU the
+    backward uses the forward's sample record) against the C ABI's backward."""
     import torch
 
     from lightmetrica_amd.autograd import render as drender
@@ -173,22 +176,22 @@ def test_autograd_function(gpu):
     hidden = albedo.copy()
     hidden[boxes] = scenes.hidden_box_albedo(len(boxes))
     for dev in ("cpu", "cuda"):
-        target = drender(gpu, torch.tensor(hidden, device=dev), torch.tensor(emission, device=dev), "pt.mis").detach()
+        target = drender(gpu, torch.tensor(hidden, device=dev), torch.tensor(emission, device=dev), renderer).detach()
         a = torch.tensor(albedo, device=dev, requires_grad=True)
         e = torch.tensor(emission, device=dev, requires_grad=True)
-        img = drender(gpu, a, e, "pt.mis")
+        img = drender(gpu, a, e, renderer)
         loss = ((img - target) ** 2).sum()
         loss.backward()
         gpu.set_params(albedo, emission)
-        ga, ge, _ = gpu.backward("pt.mis", (2.0 * (img - target)).detach().cpu().numpy())
-        np.testing.assert_allclose(a.grad.cpu().numpy(), ga, rtol=1e-6, atol=1e-9)
-        np.testing.assert_allclose(e.grad.cpu().numpy(), ge, rtol=1e-6, atol=1e-9)
+        ga, ge, _ = gpu.backward(renderer, (2.0 * (img - target)).detach().cpu().numpy())
+        np.testing.assert_allclose(a.grad.cpu().numpy(), ga, rtol=1e-5, atol=1e-9)
+        np.testing.assert_allclose(e.grad.cpu().numpy(), ge, rtol=1e-5, atol=1e-9)
         # A few projected gradient steps on the box reflectances lower the loss.
         opt = torch.optim.Adam([a], lr=0.05)
         losses = []
         for _ in range(8):
             opt.zero_grad()
-            l = ((drender(gpu, a, e.detach(), "pt.mis") - target) ** 2).sum()
+            l = ((drender(gpu, a, e.detach(), renderer) - target) ** 2).sum()
             l.backward()
             losses.append(l.item())
             opt.step()
@@ -196,3 +199,32 @@ def test_autograd_function(gpu):
                 a.clamp_(0.0, 1.0)
         print(f"{dev}: losses {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
         assert losses[-1] < 0.8 * losses[0]
+
+
+def test_sample_record_backward_equals_two_replays(gpu):
+    """pt with the forward's sample record: one replay instead of two, same gradients."""
+    import torch
+
+    sc = scenes.gradient_scene(48, 48, 8, num_boxes=150, max_path_vertices=9, rr_depth=3)
+    gpu.upload(sc)
+    gpu.build_bvh()
+    n = sc.render.num_samples
+    rec = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    img, _ = gpu.render("pt", film_precision="fp64", sample_record_ptr=rec.data_ptr())
+    # the record holds each sample's radiance: its per-pixel sums make the film
+    r = rec.cpu().numpy()
+    pix = r[:, 3].view(np.uint32)
+    film = np.zeros((48 * 48, 3))
+    np.add.at(film, pix, r[:, :3].astype(np.float64))
+    np.testing.assert_allclose(film.reshape(48, 48, 3) * (48 * 48 / n), img, rtol=1e-5, atol=1e-9)
+    dLdI = np.random.default_rng(9).standard_normal(img.shape).astype(np.float32)
+    a2, e2, info2 = gpu.backward("pt", dLdI)
+    a1, e1, info1 = gpu.backward("pt", dLdI, sample_record_ptr=rec.data_ptr())
+    print(f"residuals: two replays {info2['replay_residual']:.1e}, with record {info1['replay_residual']:.1e}; "
+          f"seconds {info2['seconds']:.4f} -> {info1['seconds']:.4f}", flush=True)
+    assert info1["replay_residual"] < 1e-6
+    np.testing.assert_allclose(a1, a2, rtol=1e-4, atol=1e-7 * np.abs(a2).max())
+    np.testing.assert_allclose(e1, e2, rtol=1e-4, atol=1e-7 * np.abs(e2).max())
+    from lightmetrica_amd.gpu import LmgpuError
+    with pytest.raises(LmgpuError):
+        gpu.render("pt.mis", sample_record_ptr=rec.data_ptr())
