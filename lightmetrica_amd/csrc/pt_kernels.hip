@@ -638,12 +638,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict_
                                                         const TriRecord* __restrict__ tris, Queue q,
                                                         const uint32_t* __restrict__ count, uint32_t* next,
                                                         float4* __restrict__ hits) {
-    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    __shared__ TraceShared<true> sm;
     const uint32_t n = *count;
     QueueSource src{q};
     HitSink sink{tris, hits};
     TraceCounters cnt{0, 0};
-    trace_persistent<false, false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+    trace_persistent<false, false>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
 // dL/dR of the vertex's material: Q / R per channel (the contributions still in Q all carry
@@ -857,12 +857,12 @@ struct ShadowSink {
 template <int kMode>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue sq, const uint32_t* __restrict__ count,
                                                         uint32_t* next) {
-    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    __shared__ TraceShared<false> sm;
     const uint32_t n = *count;
     ShadowSource src{sq};
     ShadowSink<kMode> sink{c, sq};
     TraceCounters cnt{0, 0};
-    trace_persistent<true, false>(c.nodes, c.tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+    trace_persistent<true, false>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
 }
 
 // Adjoint passes: the unoccluded shadow contributions into Q, and dL/dLe.
@@ -922,11 +922,11 @@ struct RaycastSink {
 
 template <class Out>
 __global__ __launch_bounds__(kTraceBlock) void k_raycast(PTConsts c, uint32_t n, uint32_t* next, Out* out) {
-    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    __shared__ TraceShared<true> sm;
     RaycastSource src{c.cam};
     RaycastSink<Out> sink{c, out};
     TraceCounters cnt{0, 0};
-    trace_persistent<false, false>(c.nodes, c.tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+    trace_persistent<false, false>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {

@@ -57,21 +57,21 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(const Node4* __re
                                                                uint32_t n, uint32_t* next, float* __restrict__ t_out,
                                                                float* __restrict__ b0_out, float* __restrict__ b1_out,
                                                                int32_t* __restrict__ prim_out) {
-    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    __shared__ TraceShared<true> sm;
     SoASource src{rays};
     ClosestSink<kRefine> sink{tris, t_out, b0_out, b1_out, prim_out};
     TraceCounters cnt{0, 0};
-    trace_persistent<false, false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+    trace_persistent<false, false>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_any(const Node4* __restrict__ nodes,
                                                            const TriRecord* __restrict__ tris, RaysSoA rays,
                                                            uint32_t n, uint32_t* next, uint8_t* __restrict__ occluded) {
-    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    __shared__ TraceShared<false> sm;
     SoASource src{rays};
     OcclusionSink sink{occluded};
     TraceCounters cnt{0, 0};
-    trace_persistent<true, false>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+    trace_persistent<true, false>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
 __global__ void k_zero_next(uint32_t* p) {
@@ -87,11 +87,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_count(const Node4* __rest
                                                              const TriRecord* __restrict__ tris, RaysSoA rays,
                                                              uint32_t n, uint32_t* next,
                                                              unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t lds_stack[kStackLDS * kTraceBlock];
+    __shared__ TraceShared<true> sm;
     SoASource src{rays};
     NullSink sink;
     TraceCounters cnt{0, 0};
-    trace_persistent<false, true>(nodes, tris, n, next, lds_stack + threadIdx.x, src, sink, cnt);
+    trace_persistent<false, true>(nodes, tris, n, next, sm, src, sink, cnt);
     // Wave-level reduction, one atomic per wave per counter.
     unsigned long long a = cnt.nodes, b = cnt.tris;
     for (int off = 32; off > 0; off >>= 1) {

@@ -8,10 +8,12 @@
 //     into and the others are pushed (closest hit), or all are pushed (any hit);
 //   * the traversal stack lives in LDS, [level][lane] so the 64 lanes of a wave hit
 //     distinct banks, and spills to per-lane private memory beyond kStackLDS levels;
-//   * boxes are tested with the reciprocal-direction slab test and the exit distance
-//     is scaled by 1 + 2*gamma(4) so rounding never culls a box the ray touches
-//     (Ize, "Robust BVH Ray Traversal", JCGT 2013; one more gamma term for the 1-ulp
-//     hardware reciprocal);
+//   * boxes are tested with the reciprocal-direction slab test in fused form, one FMA
+//     per plane (plane * inv + offset); the per-ray offsets -o * inv are widened by the
+//     rounding that form adds, and the exit distance is scaled by 1 + 2*gamma(4), so
+//     rounding never culls a box the ray touches (Ize, "Robust BVH Ray Traversal",
+//     JCGT 2013, with the absolute error of the fused form and the 1-ulp hardware
+//     reciprocal accounted for);
 //   * triangles use the watertight test of Woop, Benthin and Wald (JCGT 2013), with
 //     the FP64 fallback for edge functions that evaluate to exactly zero.
 // Acceptance follows the reference's QBVH: t strictly inside (tmin, tmax), the first
@@ -36,6 +38,7 @@ struct HitOut {
     float t;
     uint32_t rec;  // leaf-order triangle record of the winner
     uint32_t prim; // scene-order triangle index, 0xffffffff = miss
+    float b0, b1;  // FP32 barycentric weights of v1 and v2 (closest hit)
 };
 
 struct TraceCounters {
@@ -131,9 +134,45 @@ __device__ __forceinline__ void refine_fp64(const RayIn& r, float4 v0, float4 v1
     b1 = float(v);
 }
 
-// 1 + 2 gamma(4), gamma(n) = n u / (1 - n u), u = 2^-24: (plane - o) and * inv round once each
-// and the hardware reciprocal is within 1 ulp (2u).
+// 1 + 2 gamma(4), gamma(n) = n u / (1 - n u), u = 2^-24: the fused plane distance rounds once,
+// the hardware reciprocal is within 1 ulp (2u), and the threshold product rounds once.
 constexpr float kBoxRobust = 1.0f + 2.0f * 2.4e-7f;
+
+// Hardware reciprocal (v_rcp_f32, within 1 ulp) of a direction component kept away from 0.
+__device__ __forceinline__ float safe_rcp(float d) {
+    return __builtin_amdgcn_rcpf(fabsf(d) > 1e-30f ? d : copysignf(1e-30f, d));
+}
+
+// Per-ray constants of the slab test.  A plane at coordinate x is reached at
+// t = fma(x, inv, offset).  The offset fma(-o, inv, -+e) is -o * inv -+ e rounded once, so
+// fma(x, inv, offset) equals (x - o) * inv -+ e up to one rounding of the result and an
+// absolute error of about u |o * inv| (the offset's rounding); e = 4u |p| = 2^-22 |p|, with
+// p = o * inv, exceeds it, so moving the plane that is entered first (lo when inv >= 0)
+// earlier by e and the plane that is left last later by e leaves the entry and exit distances
+// with relative errors only, which kBoxRobust covers.  Coordinates are assumed below 1e8 in
+// magnitude so that o * inv stays finite for the clamped reciprocal (<= 1e30).
+struct Slab {
+    float ix, iy, iz;
+    float lx, hx, ly, hy, lz, hz;  // offsets of the lo and hi planes per axis
+};
+
+__device__ __forceinline__ void slab_offsets(float o, float inv, float& lo, float& hi) {
+    const float e = fabsf(o * inv) * 2.384185791015625e-7f;  // 2^-22 |p|
+    const float s = inv >= 0.f ? e : -e;
+    lo = fmaf(-o, inv, -s);
+    hi = fmaf(-o, inv, s);
+}
+
+__device__ __forceinline__ Slab make_slab(const RayIn& r) {
+    Slab s;
+    s.ix = safe_rcp(r.dx);
+    s.iy = safe_rcp(r.dy);
+    s.iz = safe_rcp(r.dz);
+    slab_offsets(r.ox, s.ix, s.lx, s.hx);
+    slab_offsets(r.oy, s.iy, s.ly, s.hy);
+    slab_offsets(r.oz, s.iz, s.lz, s.hz);
+    return s;
+}
 
 __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
     const bool s = kb < ka;
@@ -147,13 +186,12 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32
 
 constexpr uint32_t kDone = 0xffffffffu;
 
-// Slab tests of the four children of inner node `cur`.  Returns how many the ray enters
-// within [tmin, tmax]; with kSort the hit children come first in (key, val), nearest first.
+// Slab tests of the four children of the node whose words `np` points to.  Returns how many
+// the ray enters within [tmin, tmax]; with kSort the hit children come first in (key, val),
+// nearest first.
 template <bool kSort>
-__device__ __forceinline__ int node_children(const Node4* __restrict__ nodes, uint32_t cur, const RayIn& r, float ix,
-                                             float iy, float iz, float tmin, float tmax, float key[4],
+__device__ __forceinline__ int node_children(const float4* np, const Slab& s, float tmin, float tmax, float key[4],
                                              uint32_t val[4]) {
-    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
     const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
     const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
     const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
@@ -163,9 +201,9 @@ __device__ __forceinline__ int node_children(const Node4* __restrict__ nodes, ui
     int nhit = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const float t0x = (lx[i] - r.ox) * ix, t1x = (hx[i] - r.ox) * ix;
-        const float t0y = (ly[i] - r.oy) * iy, t1y = (hy[i] - r.oy) * iy;
-        const float t0z = (lz[i] - r.oz) * iz, t1z = (hz[i] - r.oz) * iz;
+        const float t0x = fmaf(lx[i], s.ix, s.lx), t1x = fmaf(hx[i], s.ix, s.hx);
+        const float t0y = fmaf(ly[i], s.iy, s.ly), t1y = fmaf(hy[i], s.iy, s.hy);
+        const float t0z = fmaf(lz[i], s.iz, s.lz), t1z = fmaf(hz[i], s.iz, s.hz);
         const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
         const bool hit = c[i] != kEmptyChild && tn <= tf * kBoxRobust;
@@ -181,6 +219,21 @@ __device__ __forceinline__ int node_children(const Node4* __restrict__ nodes, ui
         cswap(key[1], val[1], key[2], val[2]);
     }
     return nhit;
+}
+
+// Records an accepted triangle in h; closest hit also keeps the watertight test's barycentric
+// weights of v1 and v2 (V / det, W / det), so the winner is never fetched again.
+template <bool kAny>
+__device__ __forceinline__ void accept_hit(HitOut& h, float t, uint32_t rec, uint32_t prim, float V, float W,
+                                           float det) {
+    h.t = t;
+    h.rec = rec;
+    h.prim = prim;
+    if (!kAny) {
+        const float inv = 1.f / det;
+        h.b0 = V * inv;
+        h.b1 = W * inv;
+    }
 }
 
 // Triangles of leaf reference `cur`.  Accepted hits shrink tmax and update h; with kAny the
@@ -199,50 +252,105 @@ __device__ __forceinline__ bool leaf_hits(const TriRecord* __restrict__ tris, ui
         if (watertight(sh, v0, v1, v2, tmin, tmax, t, U, V, W, det)) {
             any = true;
             tmax = t;
-            h.t = t;
-            h.rec = first + j;
-            h.prim = __float_as_uint(v0.w);
+            accept_hit<kAny>(h, t, first + j, __float_as_uint(v0.w), V, W, det);
             if (kAny) break;
         }
     }
     return any;
 }
 
-// Hardware reciprocal (v_rcp_f32, within 1 ulp) of a direction component kept away from 0.
-__device__ __forceinline__ float safe_rcp(float d) {
-    return __builtin_amdgcn_rcpf(fabsf(d) > 1e-30f ? d : copysignf(1e-30f, d));
-}
+// The LDS of one traversal block: the stacks' LDS levels ([level][thread]) and, for closest
+// hit, their keys.
+template <bool kKeys>
+struct TraceShared {
+    uint32_t ref[kStackLDS * kTraceBlock];
+    uint16_t key[kKeys ? kStackLDS * kTraceBlock : 1];
+};
 
-// The per-lane traversal stack: levels [0, kStackLDS) live in LDS (`lds` is this lane's
-// column of the block's __shared__ array, lds[level * kTraceBlock]), deeper levels in
-// private memory.  The builder bounds 3 * depth <= kStackTotal, so the guard never fires.
-// The spill levels are 64-bit slots so the compiler cannot merge the two loads of pop()
-// into one generic (flat) load through a selected pointer.
-#define LMGPU_TRACE_STACK(lds)                                                  \
-    uint64_t spill[kStackPriv];                                                 \
-    int sp = 0;                                                                 \
-    auto push = [&](uint32_t v) {                                               \
-        if (sp < kStackLDS) (lds)[sp * kTraceBlock] = v;                        \
-        else if (sp < kStackTotal) spill[sp - kStackLDS] = v;                   \
-        else return;                                                            \
-        sp++;                                                                   \
-    };                                                                          \
-    auto pop = [&]() -> uint32_t {                                              \
-        sp--;                                                                   \
-        if (sp < kStackLDS) return (lds)[sp * kTraceBlock];                     \
-        return uint32_t(spill[sp - kStackLDS]);                                 \
-    };
+// Entry-distance keys of closest-hit stack entries in LDS: the upper 16 bits of the float,
+// i.e. the distance rounded toward zero, so a key never exceeds the distance it stands for;
+// distances <= 0 are kept as -inf.
+__device__ __forceinline__ uint16_t key_down16(float t) {
+    return t > 0.f ? uint16_t(__float_as_uint(t) >> 16) : uint16_t(0xff80u);
+}
+__device__ __forceinline__ float key_up(uint16_t k) { return __uint_as_float(uint32_t(k) << 16); }
+
+// The per-lane traversal stack: levels [0, kStackLDS) live in LDS (`ref` and `key` are this
+// lane's columns of the block's __shared__ arrays, ref[level * kTraceBlock]), deeper levels in
+// private memory as 64-bit slots (reference, and the full float key in the upper half).  The
+// builder bounds 3 * depth <= kStackTotal, so the guard in push never fires.
+// With kKeys (closest hit) every entry carries the entry distance of its box, and pop skips
+// entries whose distance exceeds the current tmax * kBoxRobust: the box test would reject
+// them now that a nearer hit shrank tmax, so they are never fetched.
+// The spill array is owned by the caller (a separate private array, so the stack pointer and
+// the LDS columns stay in registers).
+template <bool kKeys>
+struct TraceStack {
+    uint32_t* ref;
+    uint16_t* key;
+    uint64_t* spill;
+    int sp = 0;
+
+    __device__ __forceinline__ void push(uint32_t v, float k) {
+        if (sp < kStackLDS) {
+            ref[sp * kTraceBlock] = v;
+            if (kKeys) key[sp * kTraceBlock] = key_down16(k);
+        } else if (sp < kStackTotal) {
+            spill[sp - kStackLDS] = kKeys ? uint64_t(v) | (uint64_t(__float_as_uint(k)) << 32) : uint64_t(v);
+        } else {
+            return;
+        }
+        sp++;
+    }
+    // The next entry to visit, or kDone when the stack is empty.
+    __device__ __forceinline__ uint32_t pop(float tmax) {
+        while (sp > 0) {
+            sp--;
+            uint32_t v;
+            float k;
+            if (sp < kStackLDS) {
+                v = ref[sp * kTraceBlock];
+                k = kKeys ? key_up(key[sp * kTraceBlock]) : -INFINITY;
+            } else {
+                const uint64_t e = spill[sp - kStackLDS];
+                v = uint32_t(e);
+                k = __uint_as_float(uint32_t(e >> 32));
+            }
+            if (!kKeys || !(k > tmax * kBoxRobust)) return v;
+        }
+        return kDone;
+    }
+    // Pushes v3 (n > 3), v2 (n > 2), v1 (n > 1) in that order, with their keys.  While three
+    // LDS levels are free it stores three slots without branches and raises sp by n - 1;
+    // slots above the new top are free levels of this lane.
+    __device__ __forceinline__ void push_sorted(int n, uint32_t v1, uint32_t v2, uint32_t v3, float k1, float k2,
+                                                float k3) {
+        if (sp + 3 <= kStackLDS) {
+            ref[sp * kTraceBlock] = n > 3 ? v3 : (n > 2 ? v2 : v1);
+            ref[(sp + 1) * kTraceBlock] = n > 3 ? v2 : v1;
+            ref[(sp + 2) * kTraceBlock] = v1;
+            if (kKeys) {
+                key[sp * kTraceBlock] = key_down16(n > 3 ? k3 : (n > 2 ? k2 : k1));
+                key[(sp + 1) * kTraceBlock] = key_down16(n > 3 ? k2 : k1);
+                key[(sp + 2) * kTraceBlock] = key_down16(k1);
+            }
+            sp += n - 1;
+        } else {
+            if (n > 3) push(v3, k3);
+            if (n > 2) push(v2, k2);
+            if (n > 1) push(v1, k1);
+        }
+    }
+};
 
 // Descends from the node on top: push the farther hit children (or all but one for any
 // hit) and return the next reference to visit.
-template <bool kAny, class Push, class Pop>
-__device__ __forceinline__ uint32_t descend(int nhit, const float key[4], const uint32_t val[4], int sp, Push& push,
-                                            Pop& pop) {
-    if (nhit == 0) return sp > 0 ? pop() : kDone;
+template <bool kAny, class Stack>
+__device__ __forceinline__ uint32_t descend(int nhit, const float key[4], const uint32_t val[4], float tmax,
+                                            Stack& st) {
+    if (nhit == 0) return st.pop(tmax);
     if (!kAny) {
-        if (nhit > 3) push(val[3]);
-        if (nhit > 2) push(val[2]);
-        if (nhit > 1) push(val[1]);
+        st.push_sorted(nhit, val[1], val[2], val[3], key[1], key[2], key[3]);
         return val[0];
     }
     uint32_t first = kDone;
@@ -250,46 +358,10 @@ __device__ __forceinline__ uint32_t descend(int nhit, const float key[4], const 
     for (int i = 0; i < 4; i++) {
         if (key[i] != INFINITY) {
             if (first == kDone) first = val[i];
-            else push(val[i]);
+            else st.push(val[i], -INFINITY);
         }
     }
     return first;
-}
-
-// Core traversal of one ray.  kAny: stop at the first accepted triangle.  Returns true on
-// a hit; for closest hit, h.t/h.rec/h.prim describe the nearest triangle.
-template <bool kAny, bool kCount>
-__device__ __forceinline__ bool traverse(const Node4* __restrict__ nodes, const TriRecord* __restrict__ tris,
-                                         const RayIn& r, uint32_t* lds, HitOut& h, TraceCounters& cnt) {
-    LMGPU_TRACE_STACK(lds)
-    const float ix = safe_rcp(r.dx), iy = safe_rcp(r.dy), iz = safe_rcp(r.dz);
-    const Shear sh = make_shear(r);
-    float tmax = r.tmax;
-    const float tmin = r.tmin;
-    bool found = false;
-    h.prim = 0xffffffffu;
-    h.rec = 0;
-    h.t = tmax;
-    uint32_t cur = 0;  // root node
-    while (cur != kDone) {
-        // Inner nodes: descend until a leaf reference is current, or the stack runs out.
-        while (cur != kDone && !(cur & kLeafBit)) {
-            if (kCount) cnt.nodes++;
-            float key[4];
-            uint32_t val[4];
-            const int nhit = node_children<!kAny>(nodes, cur, r, ix, iy, iz, tmin, tmax, key, val);
-            cur = descend<kAny>(nhit, key, val, sp, push, pop);
-        }
-        // Leaves: test their triangles, then continue with the next stack entry.
-        while (cur != kDone && (cur & kLeafBit)) {
-            if (leaf_hits<kAny, kCount>(tris, cur, sh, tmin, tmax, h, cnt)) {
-                found = true;
-                if (kAny) return true;
-            }
-            cur = sp > 0 ? pop() : kDone;
-        }
-    }
-    return found;
 }
 
 // Lanes of a persistent traversal fetch new rays once this many lanes of the wave are idle
@@ -313,25 +385,25 @@ constexpr int kRefillIdle = LMGPU_REFILL_IDLE;
 // lane's node visits and triangle tests over all rays it traced.
 template <bool kAny, bool kCount, class Source, class Sink>
 __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes, const TriRecord* __restrict__ tris,
-                                                 uint32_t n, uint32_t* next, uint32_t* lds, Source& src, Sink& sink,
-                                                 TraceCounters& cnt) {
-    LMGPU_TRACE_STACK(lds)
+                                                 uint32_t n, uint32_t* next, TraceShared<!kAny>& sm, Source& src,
+                                                 Sink& sink, TraceCounters& cnt) {
+    uint64_t spill[kStackPriv];
+    TraceStack<!kAny> st{sm.ref + threadIdx.x, kAny ? nullptr : sm.key + threadIdx.x, spill};
     constexpr uint32_t kNone = 0xffffffffu;
     const uint32_t lane = __lane_id();
     uint32_t id = kNone;
     RayIn r{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f};
-    float ix = 0.f, iy = 0.f, iz = 0.f, tmax = 0.f;
+    float tmax = 0.f;
+    Slab sl = make_slab(r);
     Shear sh = make_shear(r);
-    HitOut h{0.f, 0u, kNone};
+    HitOut h{0.f, 0u, kNone, 0.f, 0.f};
     bool found = false;
     uint32_t cur = kDone;   // next node or leaf reference to visit
     uint32_t leaf = kDone;  // postponed leaf
     auto start = [&](uint32_t k) {
         id = k;
         r = src(k);
-        ix = safe_rcp(r.dx);
-        iy = safe_rcp(r.dy);
-        iz = safe_rcp(r.dz);
+        sl = make_slab(r);
         sh = make_shear(r);
         tmax = r.tmax;
         h.prim = kNone;
@@ -340,7 +412,7 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
         found = false;
         cur = 0;
         leaf = kDone;
-        sp = 0;
+        st.sp = 0;
     };
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t static_total = waves * 64u;
@@ -375,11 +447,12 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
                 if (kCount) cnt.nodes++;
                 float key[4];
                 uint32_t val[4];
-                const int nhit = node_children<!kAny>(nodes, cur, r, ix, iy, iz, r.tmin, tmax, key, val);
-                cur = descend<kAny>(nhit, key, val, sp, push, pop);
+                const int nhit =
+                    node_children<!kAny>(reinterpret_cast<const float4*>(nodes + cur), sl, r.tmin, tmax, key, val);
+                cur = descend<kAny>(nhit, key, val, tmax, st);
                 if (cur != kDone && (cur & kLeafBit) && leaf == kDone) {
                     leaf = cur;
-                    cur = sp > 0 ? pop() : kDone;
+                    cur = st.pop(tmax);
                 }
                 if (__ballot(leaf == kDone) == 0ull) break;
             }
@@ -396,7 +469,7 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
                 leaf = kDone;
                 if (cur != kDone && (cur & kLeafBit)) {
                     leaf = cur;
-                    cur = sp > 0 ? pop() : kDone;
+                    cur = st.pop(tmax);
                 }
             }
             if (id != kNone && cur == kDone) {
@@ -409,23 +482,17 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
     }
 }
 
-// Barycentrics of the recorded winner (weights of v1 and v2), FP32 or FP64.
+// Barycentrics of the recorded winner (weights of v1 and v2): the FP32 weights kept at
+// acceptance, or the winner re-evaluated in FP64.
 __device__ __forceinline__ void winner_barycentrics(const TriRecord* __restrict__ tris, const RayIn& r,
                                                     const HitOut& h, bool fp64, float& t, float& b0, float& b1) {
-    const float4* tp = reinterpret_cast<const float4*>(tris + h.rec);
-    const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
     t = h.t;
+    b0 = h.b0;
+    b1 = h.b1;
     if (fp64) {
-        refine_fp64(r, v0, v1, v2, t, b0, b1);
-        return;
+        const float4* tp = reinterpret_cast<const float4*>(tris + h.rec);
+        refine_fp64(r, tp[0], tp[1], tp[2], t, b0, b1);
     }
-    const Shear sh = make_shear(r);
-    float tt, U, V, W, det;
-    // Same arithmetic as during traversal, so the result is bit-identical to the accepted hit.
-    watertight(sh, v0, v1, v2, -INFINITY, INFINITY, tt, U, V, W, det);
-    const float inv = 1.f / det;
-    b0 = V * inv;
-    b1 = W * inv;
 }
 
 }  // namespace lmgpu

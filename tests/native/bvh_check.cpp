@@ -7,10 +7,12 @@
 //   1. every triangle is referenced by exactly one leaf record;
 //   2. every child box contains every triangle below that child;
 //   3. the device slab test restated in float (the operations of traverse.hpp: 1-ulp
-//      reciprocal direction, (plane - o) * inv, exit distance scaled by 1 + 2 gamma(4)) never rejects
-//      a child whose box a random ray enters (exact arithmetic), for rays from inside and
-//      around the scene;
-//   4. nor a ray aimed at a corner or an edge point of the child box (grazing rays).
+//      reciprocal direction, fma(plane, inv, offset) with the widened per-ray offsets, exit
+//      distance scaled by 1 + 2 gamma(4)) never rejects a child whose box a random ray enters
+//      (exact arithmetic), for rays from inside and around the scene;
+//   4. nor a ray aimed at a corner or an edge point of the child box (grazing rays);
+//   5. nor a ray that starts on a face of the child box (the origin on a plane, where the
+//      fused form's absolute error matters most).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -84,6 +86,29 @@ static float safe_rcp(float d) {
     return pick == 0 ? r : std::nextafter(r, pick == 1 ? INFINITY : -INFINITY);
 }
 
+// traverse.hpp's slab_offsets: -o * inv widened by 2^-22 |o * inv| toward the entered plane.
+static void slab_offsets(float o, float inv, float& lo, float& hi) {
+    const float e = std::fabs(o * inv) * 2.384185791015625e-7f;
+    const float s = inv >= 0.f ? e : -e;
+    lo = std::fma(-o, inv, -s);
+    hi = std::fma(-o, inv, s);
+}
+
+// The kernel's entry / exit distances of child i of node n (before the robust factor).
+static void slab_test(const Node4& n, int i, const float o[3], const float inv[3], float& tn, float& tf) {
+    const float* blo[3] = {n.lo_x, n.lo_y, n.lo_z};
+    const float* bhi[3] = {n.hi_x, n.hi_y, n.hi_z};
+    tn = 0.f;
+    tf = 3.4e38f;
+    for (int k = 0; k < 3; k++) {
+        float cl, ch;
+        slab_offsets(o[k], inv[k], cl, ch);
+        const float t0 = std::fma(blo[k][i], inv[k], cl), t1 = std::fma(bhi[k][i], inv[k], ch);
+        tn = std::max(tn, std::min(t0, t1));
+        tf = std::min(tf, std::max(t0, t1));
+    }
+}
+
 int main(int argc, char** argv) {
     const uint32_t nt = argc > 1 ? uint32_t(std::atoi(argv[1])) : 20000;
     const uint32_t seed = argc > 2 ? uint32_t(std::atoi(argv[2])) : 1;
@@ -124,12 +149,10 @@ int main(int argc, char** argv) {
             const float* bhi[3] = {n.hi_x, n.hi_y, n.hi_z};
             for (int i = 0; i < 4; i++) {
                 if (n.child[i] == kEmptyChild) continue;
-                float tn = 0.f, tf = 3.4e38f;
+                float tn, tf;
+                slab_test(n, i, o, inv, tn, tf);
                 double etn = 0.0, etf = INFINITY;
                 for (int k = 0; k < 3; k++) {
-                    const float t0 = (blo[k][i] - o[k]) * inv[k], t1 = (bhi[k][i] - o[k]) * inv[k];
-                    tn = std::max(tn, std::min(t0, t1));
-                    tf = std::min(tf, std::max(t0, t1));
                     // exact entry / exit of the child box
                     const double dd = double(d[k]);
                     if (dd == 0.0) {
@@ -169,12 +192,8 @@ int main(int argc, char** argv) {
                 if (!(len > 0.f)) continue;
                 for (float& x : d) x /= len;
                 const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
-                float tn = 0.f, tf = 3.4e38f;
-                for (int k = 0; k < 3; k++) {
-                    const float t0 = (blo[k][i] - o[k]) * inv[k], t1 = (bhi[k][i] - o[k]) * inv[k];
-                    tn = std::max(tn, std::min(t0, t1));
-                    tf = std::min(tf, std::max(t0, t1));
-                }
+                float tn, tf;
+                slab_test(n, i, o, inv, tn, tf);
                 grazing++;
                 CHECK(tn <= tf * kBoxRobust, "grazing ray at node %zu child %d rejected: [%g, %g]\n", ni, i, double(tn),
                       double(tf));
@@ -182,6 +201,36 @@ int main(int argc, char** argv) {
         }
     }
     std::printf("grazing rays %llu\n", (unsigned long long)grazing);
+    // 5. Origins on a face of the exact child box (one coordinate equal to a plane, the others
+    //    inside the face, some exactly on an edge), random directions: the ray starts in the
+    //    closed box, so the test must accept it.
+    uint64_t on_face = 0;
+    for (size_t ni = 0; ni < w.nodes.size(); ni++) {
+        const Node4& n = w.nodes[ni];
+        const float* blo[3] = {n.lo_x, n.lo_y, n.lo_z};
+        const float* bhi[3] = {n.hi_x, n.hi_y, n.hi_z};
+        for (int i = 0; i < 4; i++) {
+            if (n.child[i] == kEmptyChild) continue;
+            for (int rep = 0; rep < 4; rep++) {
+                float o[3];
+                const int face_axis = int(rng() % 3);
+                for (int k = 0; k < 3; k++) {
+                    if (k == face_axis || rng() % 4 == 0) o[k] = (rng() & 1) ? bhi[k][i] : blo[k][i];
+                    else o[k] = std::min(bhi[k][i], blo[k][i] + U(rng) * (bhi[k][i] - blo[k][i]));
+                }
+                const float z = 1.f - 2.f * U(rng), ph = 6.2831853f * U(rng), s = std::sqrt(std::max(0.f, 1.f - z * z));
+                float d[3] = {s * std::cos(ph), s * std::sin(ph), z};
+                if (rep == 3) d[rng() % 3] = 0.f;  // a ray in the face's plane
+                const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
+                float tn, tf;
+                slab_test(n, i, o, inv, tn, tf);
+                on_face++;
+                CHECK(tn <= tf * kBoxRobust, "ray from a face of node %zu child %d rejected: [%g, %g]\n", ni, i,
+                      double(tn), double(tf));
+            }
+        }
+    }
+    std::printf("rays from faces %llu\n", (unsigned long long)on_face);
     std::printf("nodes %zu, leaves %u, depth %u, box tests %llu (%llu entering), failures %d\n", w.nodes.size(),
                 w.num_leaves, w.max_depth, (unsigned long long)tested, (unsigned long long)entered, failures);
     std::printf(failures ? "bvh_check: FAILED\n" : "bvh_check: OK\n");
