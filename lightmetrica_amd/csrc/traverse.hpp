@@ -8,6 +8,8 @@
 //     into and the others are pushed (closest hit), or all are pushed (any hit);
 //   * the traversal stack lives in LDS, [level][lane] so the 64 lanes of a wave hit
 //     distinct banks, and spills to per-lane private memory beyond kStackLDS levels;
+//     closest-hit entries carry their box's entry distance, and pops skip the boxes that
+//     lie beyond the nearest hit found since they were pushed;
 //   * boxes are tested with the reciprocal-direction slab test in fused form, one FMA
 //     per plane (plane * inv + offset); the per-ray offsets -o * inv are widened by the
 //     rounding that form adds, and the exit distance is scaled by 1 + 2*gamma(4), so
