@@ -8,10 +8,12 @@ the sphere), 4-wide SAH BVH, FP32 watertight test.  One step = one closest-hit p
 over the 16M rays, with rays, BVH and outputs resident in HBM.
 
 Also reported (key "path_tracing"): the room scene (256 x 256, 16 spp, max depth 8,
-RR from depth 3) rendered with the `pt` estimator -- M path-samples/s and Mrays/s; and
-(key "adjoint") the inverse-rendering workload -- the room plus 2,000 boxes (96k
-triangles), 512 x 512 x 32 spp, max depth 8, L2 loss against a target rendered with hidden
-box reflectances -- forward and path-replay backward times and their ratio.
+RR from depth 3) rendered with the `pt` estimator -- M path-samples/s and Mrays/s -- and
+with `pt.direct` / `pt.mis` (key "renderers", closest plus any-hit rays); and (key
+"adjoint") the inverse-rendering workload -- the room plus 2,000 boxes (96k triangles),
+512 x 512 x 32 spp, max depth 8, L2 loss against a target rendered with hidden box
+reflectances -- forward and path-replay backward times, their ratio, and the adjoint
+against central differences on 64 parameters.
 
 `roofline` is measured live: HIP events around every traversal launch on the launch
 stream give the average kernel duration; algorithmic bytes per launch come from the
@@ -150,6 +152,21 @@ def run_pt(args, Device, scenes):
         _, st = dev.render("pt", film_precision="fp32")
         secs.append(st.seconds)
         rays.append(st.closest_rays)
+    # The next-event renderers on the same room: closest-hit extension rays plus any-hit
+    # shadow rays (one per non-specular vertex, the camera vertex included).
+    nee = {}
+    for renderer in ("pt.direct", "pt.mis"):
+        dev.render(renderer, num_samples=sc.render.num_samples // 16)  # warm-up
+        rs, rr = [], []
+        for _ in range(args.pt_steps):
+            _, st = dev.render(renderer, film_precision="fp32")
+            rs.append(st.seconds)
+            rr.append(st.closest_rays + st.shadow_rays)
+        m = sum(rs) / len(rs)
+        nee[renderer] = {"M_path_samples_per_s": round(sc.render.num_samples / m / 1e6, 1),
+                         "Mrays_per_s": round(sum(rr) / len(rr) / m / 1e6, 1),
+                         "ms_per_render": round(m * 1e3, 3),
+                         "rays_per_sample": round(sum(rr) / len(rr) / sc.render.num_samples, 3)}
     dev.close()
     s = sum(secs) / len(secs)
     return {
@@ -159,6 +176,7 @@ def run_pt(args, Device, scenes):
         "Mrays_per_s": (sum(rays) / len(rays)) / s / 1e6,
         "ms_per_render": s * 1e3,
         "rays_per_sample": (sum(rays) / len(rays)) / sc.render.num_samples,
+        "renderers": nee,
     }, sc
 
 
@@ -382,15 +400,15 @@ def cpu_baseline_traversal(sc, rays, sample_rays):
                       f"default scene), OpenMP over {threads} threads of {cpu_info()}; BVH build excluded"}
 
 
-def cpu_baseline_pt(sc, samples):
+def cpu_baseline_pt(sc, samples, renderer="pt"):
     from oracle import lmref
 
     threads = host_threads()
     o = lmref.Oracle(sc, fp64=False, accel="qbvh", approx_rsqrt=True)
-    _, st = o.render("pt", num_samples=samples, rng="mt", threads=threads)
+    _, st = o.render(renderer, num_samples=samples, rng="mt", threads=threads)
     return {"value": st["samples"] / st["seconds"] / 1e6, "unit": "M path-samples/s", "cores": threads,
             "kind": "port", "Mrays_per_s": st["rays"] / st["seconds"] / 1e6,
-            "sample": f"{samples} path samples of the room scene (pt, mt scheduler, 100 samples per block, "
+            "sample": f"{samples} path samples of the room scene ({renderer}, mt scheduler, 100 samples per block, "
                       f"std::mt19937 streams), {threads} threads of {cpu_info()}"}
 
 
@@ -539,6 +557,9 @@ def main():
             line["path_tracing"]["equal_rmse"] = equal_rmse_check(room, Device)
             line["path_tracing"]["speedup_vs_cpu"] = round(
                 line["path_tracing"]["M_path_samples_per_s"] / line["path_tracing"]["cpu_baseline"]["value"], 1)
+            for renderer, r in line["path_tracing"]["renderers"].items():
+                r["cpu_baseline"] = cpu_baseline_pt(room, args.cpu_sample_paths // 4, renderer)
+                r["speedup_vs_cpu"] = round(r["M_path_samples_per_s"] / r["cpu_baseline"]["value"], 1)
         line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     if rank == 0:
         print(json.dumps(line), flush=True)

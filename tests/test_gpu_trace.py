@@ -136,6 +136,26 @@ def test_tmin_excludes_near_hits(gpu):
     assert np.allclose(t2, 2.0) and np.all(prim2 >= 2)  # plane z = -1 next
 
 
+def test_negative_tmin_reaches_behind_the_origin(gpu):
+    """tmin < 0 admits hits behind the origin: the nearest t in (tmin, tmax) wins, so a ray
+    between the planes z = 0 and z = -1 heading down finds z = 0 at t = -0.5 (negative entry
+    distances on the stack are never culled), and with tmin = 0 the plane z = -1 at t = 0.5."""
+    gpu.upload(scenes.stub_simple_mesh())
+    gpu.build_bvh()
+    g = np.linspace(0.05, 0.95, 9, dtype=np.float32)
+    xy = np.stack(np.meshgrid(g, g), -1).reshape(-1, 2)
+    n = len(xy)
+    o = np.concatenate([xy, np.full((n, 1), -0.5, np.float32)], 1).astype(np.float32)
+    d = np.tile(np.array([[0, 0, -1]], np.float32), (n, 1))
+    tmax = np.full(n, scenes.FLT_MAX, np.float32)
+    t, _, _, prim = gpu.trace_closest(scenes.Rays(o, d, np.full(n, -1.0, np.float32), tmax))
+    np.testing.assert_allclose(t, -0.5, rtol=1e-6)
+    assert np.all((prim >= 0) & (prim < 2))  # the z = 0 square
+    t, _, _, prim = gpu.trace_closest(scenes.Rays(o, d, np.zeros(n, np.float32), tmax))
+    np.testing.assert_allclose(t, 0.5, rtol=1e-6)
+    assert np.all(prim >= 2)  # the z = -1 square
+
+
 def test_empty_and_tiny_batches(gpu):
     gpu.upload(scenes.stub_simple_mesh())
     gpu.build_bvh()
