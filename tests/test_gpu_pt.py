@@ -67,6 +67,56 @@ def test_env_scene_same_streams(gpu, oracle, renderer):
     same_stream_check(g, o, f"env/{renderer}")
 
 
+def random_scene(case: int, res=32, spp=16):
+    """A random open scene: a ground quad, 1-6 rotated boxes with random diffuse / mirror /
+    diffuse-mirror materials, 1-3 area lights of random emission, a constant environment
+    in half of the cases, a random camera and random path-length settings."""
+    rng = np.random.default_rng(1000 + case)
+    b = scenes._Builder()
+    entries = [(scenes.BSDF_DIFFUSE, tuple(rng.uniform(0.2, 0.9, 3)))]
+    b.quad((-2, 0, -2), (0, 0, 4), (4, 0, 0), 0)
+    for _ in range(int(rng.integers(1, 7))):
+        kind = int(rng.choice([scenes.BSDF_DIFFUSE, scenes.BSDF_MIRROR, scenes.BSDF_DIFFUSEMIRROR]))
+        entries.append((kind, tuple(rng.uniform(0.1, 0.95, 3))))
+        size = rng.uniform(0.1, 0.7, 3)
+        center = (rng.uniform(-0.8, 0.8), size[1] / 2, rng.uniform(-0.8, 0.8))
+        b.box(center, size, float(rng.uniform(-45, 45)), len(entries) - 1)
+    light_entries = []
+    black = len(entries)
+    entries.append((scenes.BSDF_DIFFUSE, (0.0, 0.0, 0.0)))
+    for _ in range(int(rng.integers(1, 4))):
+        s = rng.uniform(0.2, 0.8)
+        corner = (rng.uniform(-1.0, 0.5), rng.uniform(1.2, 2.0), rng.uniform(-1.0, 0.5))
+        b.quad(corner, (s, 0, 0), (0, 0, s), black, light=len(light_entries))
+        light_entries.append((scenes.LIGHT_AREA, tuple(rng.uniform(2.0, 12.0, 3))))
+    if rng.random() < 0.5:
+        light_entries.append((scenes.LIGHT_ENV_CONST, tuple(rng.uniform(0.1, 0.8, 3))))
+    P, N, F, M, L = b.arrays()
+    eye = (rng.uniform(-1.5, 1.5), rng.uniform(0.8, 2.0), rng.uniform(2.0, 3.0))
+    view = scenes.look_at(eye, (0.0, 0.3, 0.0), (0, 1, 0)).astype(np.float32)
+    cam = scenes.Camera(view=view, fovy=float(rng.uniform(35, 70)), width=res, height=res)
+    max_pv = int(rng.integers(3, 11))
+    rc = scenes.RenderConfig(num_samples=res * res * spp, max_path_vertices=max_pv,
+                             rr_depth=int(rng.integers(1, max_pv + 1)), seed=int(rng.integers(1, 1 << 30)))
+    return scenes.Scene(f"random{case}", P, N, F, M, L, scenes._materials(entries), scenes._lights(light_entries),
+                        cam, rc)
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_random_scenes_same_streams(gpu, oracle, case):
+    """Random scenes and settings, every renderer: the GPU and the FP64 oracle trace the same
+    streams to the same image."""
+    sc = random_scene(case)
+    gpu.upload(sc)
+    gpu.build_bvh()
+    o = oracle.Oracle(sc, fp64=True, accel="bvh")
+    for renderer in RENDERERS:
+        g, _ = gpu.render(renderer, film_precision="fp64")
+        ref, _ = o.render(renderer)
+        same_stream_check(g, ref, f"random{case}/{renderer} ({sc.num_triangles} tris, "
+                                  f"max_pv {sc.render.max_path_vertices}, rr {sc.render.rr_depth})")
+
+
 def test_fp32_film_matches_fp64_film(gpu):
     sc = scenes.room_scene(64, 64, 16)
     gpu.upload(sc)
