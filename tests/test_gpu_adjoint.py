@@ -124,6 +124,35 @@ def test_adjoint_matches_oracle_central_differences(gpu, oracle, renderer):
     compare(adjoint_at(ga, ge, params), g_fd, f"{renderer} vs oracle FD", 2e-2)
 
 
+@pytest.mark.parametrize("case", [0, 3, 5, 6, 10])
+def test_random_scenes_adjoint_matches_central_differences(gpu, case):
+    """The random scenes of the same-stream parity test (mirror and diffuse-mirror boxes,
+    several area lights, an environment in some), Russian roulette off: the adjoint of every
+    renderer against central differences of the GPU forward over every non-zero reflectance
+    channel and every emission channel."""
+    from test_gpu_pt import random_scene
+
+    sc = random_scene(case)
+    sc.render.rr_depth = sc.render.max_path_vertices
+    gpu.upload(sc)
+    gpu.build_bvh()
+    albedo, emission = base_params(sc)
+    rng = np.random.default_rng(case)
+    target_a = np.clip(albedo * rng.uniform(0.7, 1.3, albedo.shape), 0.0, 1.0)
+    params = [("albedo", i, c) for i in range(len(albedo)) for c in range(3) if albedo[i, c] > 0]
+    params += [("emission", i, c) for i in range(len(emission)) for c in range(3)]
+    for renderer in RENDERERS:
+        target = render(gpu, renderer, target_a, emission * 1.1)
+        img = render(gpu, renderer, albedo, emission)
+        _, dLdI = loss_and_grad(img, target)
+        gpu.set_params(albedo, emission)
+        ga, ge, info = gpu.backward(renderer, dLdI)
+        assert info["replay_residual"] < 1e-9
+        g_fd = fd_gradient(lambda a, e: render(gpu, renderer, a, e), albedo, emission, target, params)
+        compare(adjoint_at(ga, ge, params), g_fd, f"random{case}/{renderer} vs GPU FD ({len(params)} parameters)",
+                2e-3)
+
+
 @pytest.mark.parametrize("renderer", RENDERERS)
 def test_replay_is_exact_with_russian_roulette(gpu, renderer):
     """With Russian roulette on (its probabilities held constant) the replay still retraces
