@@ -353,6 +353,14 @@ def run_city(args, torch, Device, scenes):
     ga, ge, binfo = dev.backward("pt", img.astype(np.float32))
     log(f"[city] backward in {binfo['seconds']:.2f}s, replay residual {binfo['replay_residual']:.1e}")
     free2, _ = torch.cuda.mem_get_info(0)
+    # The same with the forward's per-sample record (16 B per sample, resident in HBM): one replay.
+    record = torch.empty((sc.render.num_samples, 4), dtype=torch.float32, device="cuda:0")
+    img_r, st_r = dev.render("pt", film_precision="fp32", out_precision="fp32", sample_record_ptr=record.data_ptr())
+    ga_r, ge_r, rinfo = dev.backward("pt", img_r.astype(np.float32), sample_record_ptr=record.data_ptr())
+    free3, _ = torch.cuda.mem_get_info(0)
+    log(f"[city] forward with record {st_r.seconds:.2f}s, backward from the record {rinfo['seconds']:.2f}s")
+    rec_agree = float(np.abs(ga_r - ga).max() / max(np.abs(ga).max(), 1e-30))
+    del record
     dev.close()
     res = {"workload": f"city: 1000x1000 boxes (16,000,034 triangles), log-normal heights, 10k albedo groups, "
                        f"16 quad lights; 3840x2160, {spp} spp ({sc.render.num_samples} samples), max_path_vertices 13, "
@@ -364,10 +372,16 @@ def run_city(args, torch, Device, scenes):
            "backward_s": round(binfo["seconds"], 3),
            "backward_over_forward": round(binfo["seconds"] / st.seconds, 3),
            "replay_residual": binfo["replay_residual"],
+           "forward_with_record_s": round(st_r.seconds, 3),
+           "backward_from_record_s": round(rinfo["seconds"], 3),
+           "backward_from_record_over_forward": round(rinfo["seconds"] / st_r.seconds, 3),
+           "record_GiB": round(sc.render.num_samples * 16 / 2**30, 2),
+           "gradient_rel_diff_record_vs_replays": rec_agree,
            "bvh_build_s": round(info.build_seconds, 2),
            "bvh_device_GiB": round(info.device_bytes / 2**30, 3),
            "hbm_used_forward_GiB": round((free0 - free1) / 2**30, 3),
            "hbm_used_peak_GiB": round((free0 - min(free1, free2)) / 2**30, 3),
+           "hbm_used_peak_with_record_GiB": round((free0 - free3) / 2**30, 3),
            "hbm_total_GiB": round(total / 2**30, 1)}
     if not args.no_cpu_baseline:
         from oracle import lmref
