@@ -10,8 +10,9 @@ backward runs lmgpu_pt_backward with the same sample streams, so the gradient is
 the very estimate the forward returned.  GPU tensors are passed as device pointers and
 the context is switched to torch's current stream for the call, so no host copy of the
 image or its gradient is made.  For renderer "pt" on the GPU the forward also records
-each sample's radiance (16 bytes per sample, up to RECORD_LIMIT bytes), so the backward
-replays the paths once instead of twice.
+each sample's radiance (16 bytes per sample) when that fits in RECORD_LIMIT bytes -- by
+default half of the device memory free at the forward -- so the backward replays the paths
+once instead of twice.
 """
 from __future__ import annotations
 
@@ -22,7 +23,7 @@ from .gpu import Device
 
 __all__ = ["PTRender", "render", "RECORD_LIMIT"]
 
-RECORD_LIMIT = 4 << 30  # bytes of per-sample record the forward may allocate
+RECORD_LIMIT = None  # bytes of per-sample record the forward may allocate; None: half the free HBM
 
 
 def _params(t: torch.Tensor) -> np.ndarray:
@@ -40,7 +41,8 @@ class PTRender(torch.autograd.Function):
         if on_gpu:
             img = torch.empty(shape, dtype=torch.float32, device=albedo.device)
             n = int(opts.get("num_samples") or sc.render.num_samples)
-            if renderer == "pt" and 16 * n <= RECORD_LIMIT and (albedo.requires_grad or emission.requires_grad):
+            limit = RECORD_LIMIT if RECORD_LIMIT is not None else torch.cuda.mem_get_info(albedo.device)[0] // 2
+            if renderer == "pt" and 16 * n <= limit and (albedo.requires_grad or emission.requires_grad):
                 record = torch.empty((n, 4), dtype=torch.float32, device=albedo.device)
             device.set_stream(torch.cuda.current_stream(albedo.device).cuda_stream)
             try:
