@@ -212,6 +212,28 @@ def run_adjoint(args, Device, scenes):
         _, _, info2 = dev.backward("pt", d)
         bwd2.append(info2["seconds"])
     fd = adjoint_fd_check(dev, sc, albedo, emission, hidden, boxes)
+    # The next-event renderers on the same workload, with their two-slot records.
+    from lightmetrica_amd.gpu import record_float4s
+
+    nee = {}
+    for renderer in ("pt.direct", "pt.mis"):
+        dev.set_params(hidden, emission)
+        tgt, _ = dev.render(renderer, film_precision="fp32")
+        dev.set_params(albedo, emission)
+        rec = torch.empty((record_float4s(renderer, sc.render.num_samples), 4), dtype=torch.float32, device="cuda:0")
+        rf, rb, rb2 = [], [], []
+        for _ in range(args.pt_steps):
+            img, st = dev.render(renderer, film_precision="fp32", sample_record_ptr=rec.data_ptr())
+            rf.append(st.seconds)
+            d = (2.0 * (img - tgt)).astype(np.float32)
+            rb.append(dev.backward(renderer, d, sample_record_ptr=rec.data_ptr())[2]["seconds"])
+            rb2.append(dev.backward(renderer, d)[2]["seconds"])
+        mf, mb, mb2 = sum(rf) / len(rf), sum(rb) / len(rb), sum(rb2) / len(rb2)
+        nee[renderer] = {"forward_ms": round(mf * 1e3, 3), "backward_ms": round(mb * 1e3, 3),
+                         "backward_over_forward": round(mb / mf, 3),
+                         "backward_without_record_ms": round(mb2 * 1e3, 3),
+                         "backward_without_record_over_forward": round(mb2 / mf, 3)}
+        del rec
     dev.close()
     f, b, b2 = sum(fwd) / len(fwd), sum(bwd) / len(bwd), sum(bwd2) / len(bwd2)
     return {
@@ -228,6 +250,7 @@ def run_adjoint(args, Device, scenes):
         "replay_residual": max(res),
         "gradient_norm": float(np.linalg.norm(ga)),
         "fd_check": fd,
+        "renderers": nee,
     }
 
 

@@ -165,9 +165,12 @@ typedef struct lmgpu_pt_params {
     uint64_t sample_offset;   /* global index of the first sample (shards of a larger render) */
     uint32_t wavefront_width; /* paths in flight per batch; 0 = auto */
     int film_precision;       /* LMGPU_FILM_* accumulation type on the device */
-    /* Optional, renderer LMGPU_PT only: a device buffer of num_samples float4.  The render
-     * writes each sample's radiance (rgb) and film pixel (w, as uint32 bits); a backward with
-     * the same buffer reads them instead of replaying the paths once more.  NULL: unused. */
+    /* Optional: a device buffer of num_samples float4 (LMGPU_PT) or 2 * num_samples float4
+     * (LMGPU_PT_DIRECT, LMGPU_PT_MIS).  The render writes each sample's contribution (rgb)
+     * and film pixel (w, as uint32 bits; 0xffffffff = none): for LMGPU_PT its radiance, for
+     * the next-event renderers the sums at the path's pixel and at the pixel of the camera
+     * vertex's light connection.  A backward with the same buffer reads them instead of
+     * replaying the paths once more.  NULL: unused. */
     void* sample_record;
 } lmgpu_pt_params;
 

@@ -9,17 +9,17 @@ context drives.  The forward uploads them (lmgpu_scene_set_params) and renders; 
 backward runs lmgpu_pt_backward with the same sample streams, so the gradient is that of
 the very estimate the forward returned.  GPU tensors are passed as device pointers and
 the context is switched to torch's current stream for the call, so no host copy of the
-image or its gradient is made.  For renderer "pt" on the GPU the forward also records
-each sample's radiance (16 bytes per sample) when that fits in RECORD_LIMIT bytes -- by
-default half of the device memory free at the forward -- so the backward replays the paths
-once instead of twice.
+image or its gradient is made.  On the GPU the forward also records each sample's
+contributions (16 bytes per sample for "pt", 32 for "pt.direct" / "pt.mis") when that fits in
+RECORD_LIMIT bytes -- by default half of the device memory free at the forward -- so the
+backward replays the paths once instead of twice.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from .gpu import Device
+from .gpu import Device, record_float4s
 
 __all__ = ["PTRender", "render", "RECORD_LIMIT"]
 
@@ -42,8 +42,9 @@ class PTRender(torch.autograd.Function):
             img = torch.empty(shape, dtype=torch.float32, device=albedo.device)
             n = int(opts.get("num_samples") or sc.render.num_samples)
             limit = RECORD_LIMIT if RECORD_LIMIT is not None else torch.cuda.mem_get_info(albedo.device)[0] // 2
-            if renderer == "pt" and 16 * n <= limit and (albedo.requires_grad or emission.requires_grad):
-                record = torch.empty((n, 4), dtype=torch.float32, device=albedo.device)
+            m = record_float4s(renderer, n)
+            if 16 * m <= limit and (albedo.requires_grad or emission.requires_grad):
+                record = torch.empty((m, 4), dtype=torch.float32, device=albedo.device)
             device.set_stream(torch.cuda.current_stream(albedo.device).cuda_stream)
             try:
                 device.render(renderer, film_precision="fp64", out_precision="fp32", device_out_ptr=img.data_ptr(),

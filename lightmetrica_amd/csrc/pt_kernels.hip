@@ -166,6 +166,24 @@ __device__ __forceinline__ void film_add(const PTConsts& c, uint32_t pixel, V3 v
     }
 }
 
+// Sample records.  pt: one float4 per sample, (radiance, pixel bits), written when the path
+// ends.  pt.direct / pt.mis: two float4 per sample, (sum of contributions, pixel bits) for at
+// most two pixels -- the path's own pixel, set by the camera stage, and the pixel of the
+// camera vertex's light connection -- which are all the pixels a sample of these estimators
+// touches.  A sample's contributions come one per launch, so the updates need no atomics.
+constexpr uint32_t kNoPixel = 0xffffffffu;
+
+__device__ __forceinline__ void record_add(const PTConsts& c, uint32_t sid, uint32_t pixel, V3 v) {
+    if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return;
+    float4* r = c.record + 2ull * sid;
+    float4 a = r[0];
+    if (__float_as_uint(a.w) != pixel) {
+        r += 1;
+        a = r[0];
+    }
+    r[0] = make_float4(a.x + v.x, a.y + v.y, a.z + v.z, __uint_as_float(pixel));
+}
+
 // One contribution v of batch sample `sid` to film pixel `pixel`: the film in the render,
 // +/- g[pixel] * v into the sample's Q in the adjoint passes.  Per launch a sample makes at
 // most one call, so the read-modify-write of its Q slot needs no atomics.
@@ -173,6 +191,7 @@ template <int kMode>
 __device__ __forceinline__ void contribute(const PTConsts& c, uint32_t sid, uint32_t pixel, V3 v) {
     if (kMode == kRender) {
         film_add(c, pixel, v);
+        if (c.record) record_add(c, sid, pixel, v);
         return;
     }
     if (is_zero(v)) return;
@@ -609,6 +628,10 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
     const uint32_t slot = block_append<1>(live_count, alive);
     if (alive)
         store_path(out, slot, cam_p, kEps, d, pdf, T, v3s(0.f), i, 1u | (kFlagDegenerate << 16), rng.dim, pixel);
+    if (kMode == kRender && c.record && valid) {
+        c.record[2ull * i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(alive ? pixel : kNoPixel));
+        c.record[2ull * i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kNoPixel));
+    }
 }
 
 // Extension rays: closest hit of every live path (persistent, refilling lanes).
@@ -850,7 +873,10 @@ struct ShadowSink {
         if (found) return;
         const float4 v = sq.c[i];
         const V3 Le = light_Le(c, __float_as_int(v.w));
-        film_add(c, __float_as_uint(sq.d[i].w), v3(v.x, v.y, v.z) * Le);
+        const uint32_t pixel = __float_as_uint(sq.d[i].w);
+        const V3 x = v3(v.x, v.y, v.z) * Le;
+        film_add(c, pixel, x);
+        if (c.record) record_add(c, sq.sid[i], pixel, x);
     }
 };
 
@@ -980,16 +1006,25 @@ __global__ __launch_bounds__(256) void k_abs_sum(const double* __restrict__ q, u
     if (threadIdx.x == 0) unsafeAtomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
-// Adjoint pass 1 from a pt sample record: Q = dL/dI[pixel] * radiance.
-__global__ void k_q_from_record(const float4* __restrict__ rec, const float* __restrict__ dLdI, uint32_t n,
-                                double* __restrict__ q) {
+// Adjoint pass 1 from a sample record (`stride` float4 per sample):
+// Q = sum over the record's pixels of dL/dI[pixel] * contribution.
+__global__ void k_q_from_record(const float4* __restrict__ rec, uint32_t stride, const float* __restrict__ dLdI,
+                                uint32_t n, double* __restrict__ q) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 r = rec[i];
-    const float* g = dLdI + 3ull * __float_as_uint(r.w);
-    q[4ull * i + 0] = double(g[0]) * double(r.x);
-    q[4ull * i + 1] = double(g[1]) * double(r.y);
-    q[4ull * i + 2] = double(g[2]) * double(r.z);
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (uint32_t k = 0; k < stride; k++) {
+        const float4 r = rec[size_t(stride) * i + k];
+        const uint32_t pixel = __float_as_uint(r.w);
+        if (pixel == kNoPixel) continue;
+        const float* g = dLdI + 3ull * pixel;
+        x += double(g[0]) * double(r.x);
+        y += double(g[1]) * double(r.y);
+        z += double(g[2]) * double(r.z);
+    }
+    q[4ull * i + 0] = x;
+    q[4ull * i + 1] = y;
+    q[4ull * i + 2] = z;
     q[4ull * i + 3] = 0.0;
 }
 
@@ -1056,8 +1091,6 @@ static lmgpu_status validate(lmgpu_ctx* ctx, const lmgpu_pt_params* p, const cha
         return fail(ctx, LMGPU_ERR_UNSUPPORTED, f + ": max_path_vertices must be in [1, 64]");
     if (3u * ctx->bvh_max_depth > uint32_t(kStackTotal))
         return fail(ctx, LMGPU_ERR_BVH_TOO_DEEP, f + ": BVH deeper than the traversal stack");
-    if (p->sample_record && p->renderer != LMGPU_PT)
-        return fail(ctx, LMGPU_ERR_UNSUPPORTED, f + ": a sample record needs renderer LMGPU_PT");
     return LMGPU_OK;
 }
 
@@ -1264,8 +1297,9 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
     hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((acc_bytes / 16 + 255) / 256)), dim3(256), 0, s,
                        static_cast<uint4*>(ctx->film_acc), acc_bytes / 16);
     hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, pl.b.counters, (4096u + 64u) / 4u);
+    const uint32_t rec_stride = p->renderer == LMGPU_PT ? 1u : 2u;  // float4 per sample in the record
     for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap) {
-        pl.c.record = p->sample_record ? static_cast<float4*>(p->sample_record) + base : nullptr;
+        pl.c.record = p->sample_record ? static_cast<float4*>(p->sample_record) + rec_stride * base : nullptr;
         st = pl.batch<kRender>(p->sample_offset + base, uint32_t(std::min<uint64_t>(pl.cap, p->num_samples - base)));
     }
     if (st == LMGPU_OK) {
@@ -1368,12 +1402,14 @@ lmgpu_status run_pt_backward(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_adj
     hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t(((g_bytes + 16) / 16 + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<uint4*>(grad), (g_bytes + 16) / 16);
     hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, pl.b.counters, (4096u + 64u) / 4u);
+    const uint32_t rec_stride = p->renderer == LMGPU_PT ? 1u : 2u;  // float4 per sample in the record
     for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap) {
         const uint32_t nb = uint32_t(std::min<uint64_t>(pl.cap, p->num_samples - base));
         if (p->sample_record) {
-            // The forward's record holds every sample's radiance: Q without a replay.
+            // The forward's record holds every sample's contributions: Q without a replay.
             hipLaunchKernelGGL(k_q_from_record, dim3((nb + 255) / 256), dim3(256), 0, s,
-                               static_cast<const float4*>(p->sample_record) + base, pl.c.dLdI, nb, pl.c.qacc);
+                               static_cast<const float4*>(p->sample_record) + rec_stride * base, rec_stride,
+                               pl.c.dLdI, nb, pl.c.qacc);
             e = hipGetLastError();
             if (e != hipSuccess) {
                 st = hip_fail(ctx, e, "k_q_from_record");

@@ -23,6 +23,12 @@ FILM_PRECISIONS = {"fp32": N.LMGPU_FILM_FP32, "fp64": N.LMGPU_FILM_FP64,
 K_TRACE_CLOSEST, K_TRACE_ANY, K_SHADE, K_GEN, K_SHADOW, K_EXTEND = range(6)
 
 
+def record_float4s(renderer: str, num_samples: int) -> int:
+    """float4 entries of a sample record: one per sample for "pt", two for "pt.direct" /
+    "pt.mis" (the path's pixel and the camera vertex's light connection)."""
+    return num_samples * (1 if renderer == "pt" else 2)
+
+
 class LmgpuError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{N.STATUS_NAMES.get(status, status)}: {message}")
@@ -194,8 +200,9 @@ class Device:
         """Render the uploaded scene.  Returns (film (H, W, 3) or None for device output, RenderStats).
 
         Film rows are bottom-up (raster y = 0 is the bottom row), values rescaled by W*H/N.
-        sample_record_ptr: device buffer of num_samples float4 (renderer "pt" only) that receives
-        each sample's radiance and pixel, for a backward that skips its first replay.
+        sample_record_ptr: device buffer of record_float4s(renderer, num_samples) float4 that
+        receives each sample's contributions and pixels, for a backward that skips its first
+        replay.
         """
         sc = self.scene
         if sc is None or sc.camera is None:
@@ -262,8 +269,8 @@ class Device:
                  sample_record_ptr: Optional[int] = None):
         """Path-replay adjoint of render(renderer, ...) with the same arguments.
 
-        sample_record_ptr: the record a "pt" render wrote (same arguments); the backward then
-        takes the per-sample totals from it and replays the paths once instead of twice.
+        sample_record_ptr: the record a render wrote (same arguments); the backward then takes
+        the per-sample totals from it and replays the paths once instead of twice.
 
         d_image: dL/dI, (H, W, 3) float32 numpy array, or an int device pointer to such data.
         Returns (d_albedo (num_materials, 3), d_emission (num_lights, 3), info) -- the arrays

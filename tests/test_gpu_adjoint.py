@@ -191,8 +191,8 @@ def test_linear_loss_gradient_is_the_emission_image(gpu):
 
 @pytest.mark.parametrize("renderer", ["pt.mis", "pt"])
 def test_autograd_function(gpu, renderer):
-    """render() as a torch.autograd.Function on CPU and GPU tensors (for "pt" on the GPU the
-    backward uses the forward's sample record) against the C ABI's backward."""
+    """render() as a torch.autograd.Function on CPU and GPU tensors (on the GPU the backward
+    uses the forward's sample record) against the C ABI's backward."""
     import torch
 
     from lightmetrica_amd.autograd import render as drender
@@ -230,30 +230,36 @@ def test_autograd_function(gpu, renderer):
         assert losses[-1] < 0.8 * losses[0]
 
 
-def test_sample_record_backward_equals_two_replays(gpu):
-    """pt with the forward's sample record: one replay instead of two, same gradients."""
+@pytest.mark.parametrize("renderer", RENDERERS)
+def test_sample_record_backward_equals_two_replays(gpu, renderer):
+    """The forward's sample record (pt: radiance and pixel per sample; pt.direct / pt.mis: the
+    sums at the path's pixel and at the camera connection's pixel): its per-pixel sums make the
+    film, and a backward from it (one replay) gives the gradients of two replays."""
     import torch
+
+    from lightmetrica_amd.gpu import record_float4s
 
     sc = scenes.gradient_scene(48, 48, 8, num_boxes=150, max_path_vertices=9, rr_depth=3)
     gpu.upload(sc)
     gpu.build_bvh()
     n = sc.render.num_samples
-    rec = torch.empty((n, 4), dtype=torch.float32, device="cuda")
-    img, _ = gpu.render("pt", film_precision="fp64", sample_record_ptr=rec.data_ptr())
-    # the record holds each sample's radiance: its per-pixel sums make the film
+    rec = torch.empty((record_float4s(renderer, n), 4), dtype=torch.float32, device="cuda")
+    img, _ = gpu.render(renderer, film_precision="fp64", sample_record_ptr=rec.data_ptr())
     r = rec.cpu().numpy()
     pix = r[:, 3].view(np.uint32)
+    used = pix != 0xFFFFFFFF
+    if renderer != "pt":
+        # every sample's second slot is either unused or a pixel other than the first slot's
+        pairs = pix.reshape(-1, 2)
+        assert np.all((pairs[:, 1] == 0xFFFFFFFF) | (pairs[:, 1] != pairs[:, 0]))
     film = np.zeros((48 * 48, 3))
-    np.add.at(film, pix, r[:, :3].astype(np.float64))
+    np.add.at(film, pix[used], r[used, :3].astype(np.float64))
     np.testing.assert_allclose(film.reshape(48, 48, 3) * (48 * 48 / n), img, rtol=1e-5, atol=1e-9)
     dLdI = np.random.default_rng(9).standard_normal(img.shape).astype(np.float32)
-    a2, e2, info2 = gpu.backward("pt", dLdI)
-    a1, e1, info1 = gpu.backward("pt", dLdI, sample_record_ptr=rec.data_ptr())
-    print(f"residuals: two replays {info2['replay_residual']:.1e}, with record {info1['replay_residual']:.1e}; "
-          f"seconds {info2['seconds']:.4f} -> {info1['seconds']:.4f}", flush=True)
+    a2, e2, info2 = gpu.backward(renderer, dLdI)
+    a1, e1, info1 = gpu.backward(renderer, dLdI, sample_record_ptr=rec.data_ptr())
+    print(f"{renderer}: residuals: two replays {info2['replay_residual']:.1e}, with record "
+          f"{info1['replay_residual']:.1e}; seconds {info2['seconds']:.4f} -> {info1['seconds']:.4f}", flush=True)
     assert info1["replay_residual"] < 1e-6
     np.testing.assert_allclose(a1, a2, rtol=1e-4, atol=1e-7 * np.abs(a2).max())
     np.testing.assert_allclose(e1, e2, rtol=1e-4, atol=1e-7 * np.abs(e2).max())
-    from lightmetrica_amd.gpu import LmgpuError
-    with pytest.raises(LmgpuError):
-        gpu.render("pt.mis", sample_record_ptr=rec.data_ptr())
