@@ -666,7 +666,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict_
     QueueSource src{q};
     HitSink sink{tris, hits};
     TraceCounters cnt{0, 0};
-    trace_persistent<false, false>(nodes, tris, n, next, sm, src, sink, cnt);
+    trace_persistent<false, false, false>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
 // dL/dR of the vertex's material: Q / R per channel (the contributions still in Q all carry
@@ -888,7 +888,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue 
     ShadowSource src{sq};
     ShadowSink<kMode> sink{c, sq};
     TraceCounters cnt{0, 0};
-    trace_persistent<true, false>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
+    trace_persistent<true, false, false>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
 }
 
 // Adjoint passes: the unoccluded shadow contributions into Q, and dL/dLe.
@@ -952,7 +952,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_raycast(PTConsts c, uint32_t n,
     RaycastSource src{c.cam};
     RaycastSink<Out> sink{c, out};
     TraceCounters cnt{0, 0};
-    trace_persistent<false, false>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
+    trace_persistent<false, false, true>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {

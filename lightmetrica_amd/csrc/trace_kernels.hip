@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(const Node4* __re
     SoASource src{rays};
     ClosestSink<kRefine> sink{tris, t_out, b0_out, b1_out, prim_out};
     TraceCounters cnt{0, 0};
-    trace_persistent<false, false>(nodes, tris, n, next, sm, src, sink, cnt);
+    trace_persistent<false, false, true>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_any(const Node4* __restrict__ nodes,
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_any(const Node4* __restri
     SoASource src{rays};
     OcclusionSink sink{occluded};
     TraceCounters cnt{0, 0};
-    trace_persistent<true, false>(nodes, tris, n, next, sm, src, sink, cnt);
+    trace_persistent<true, false, true>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
 __global__ void k_zero_next(uint32_t* p) {
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_count(const Node4* __rest
     SoASource src{rays};
     NullSink sink;
     TraceCounters cnt{0, 0};
-    trace_persistent<false, true>(nodes, tris, n, next, sm, src, sink, cnt);
+    trace_persistent<false, true, true>(nodes, tris, n, next, sm, src, sink, cnt);
     // Wave-level reduction, one atomic per wave per counter.
     unsigned long long a = cnt.nodes, b = cnt.tris;
     for (int off = 32; off > 0; off >>= 1) {

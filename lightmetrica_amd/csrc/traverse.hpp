@@ -366,26 +366,27 @@ __device__ __forceinline__ uint32_t descend(int nhit, const float key[4], const 
     return first;
 }
 
-// Lanes of a persistent traversal fetch new rays once this many lanes of the wave are idle
-// (a build-time tuning knob).
-#ifndef LMGPU_REFILL_IDLE
-#define LMGPU_REFILL_IDLE 40
-#endif
-constexpr int kRefillIdle = LMGPU_REFILL_IDLE;
+// Lanes of a persistent traversal fetch new rays once this many lanes of the wave are idle:
+// kRefillIdle when every refill claims its rays with an atomic (the path tracer's queues),
+// kRefillIdleRanged when rays come from ranges claimed ahead of need (ray batches).
+constexpr int kRefillIdle = 40;
+constexpr int kRefillIdleRanged = 8;
 
 // Persistent traversal with per-lane ray refill (Aila & Laine, "Understanding the
 // efficiency of ray traversal on GPUs", HPG 2009): every wave of the grid loops over
 // the batch.  The first 64 rays of a wave are assigned statically (wave w takes
-// [64 w, 64 w + 64)); after that a lane whose ray has finished fetches the next unclaimed
-// ray, with one atomic per wave on `next` (counting from 64 x the number of waves), once
-// kRefillIdle lanes are idle.  A batch smaller than the grid therefore costs no atomics.
+// [64 w, 64 w + 64)); after that lanes whose rays have finished fetch the next unclaimed
+// rays from `next` (counting from 64 x the number of waves) once enough of the wave's lanes
+// are idle: without kRanged with one atomic per refill, at kRefillIdle idle lanes; with
+// kRanged from 64-ray ranges claimed one range ahead, at kRefillIdleRanged idle lanes.  A
+// batch smaller than the grid therefore costs no atomics.
 // Leaves are processed speculatively (same paper): a lane that reaches a leaf postpones it
 // and keeps descending until every lane still descending holds a leaf, so the inner-node
 // and triangle loops each run with as many lanes as possible.
 // `src(i)` returns ray i, `sink(i, found, h, r)` consumes its result.  Every wave exits
 // once the batch is exhausted and its own lanes are done.  With kCount, `cnt` sums the
 // lane's node visits and triangle tests over all rays it traced.
-template <bool kAny, bool kCount, class Source, class Sink>
+template <bool kAny, bool kCount, bool kRanged, class Source, class Sink>
 __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes, const TriRecord* __restrict__ tris,
                                                  uint32_t n, uint32_t* next, TraceShared<!kAny>& sm, Source& src,
                                                  Sink& sink, TraceCounters& cnt) {
@@ -424,8 +425,50 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
     }
     bool exhausted = static_total >= n;  // wave-uniform
     bool first = true;
+    // kRanged: rays come from ranges -- the wave's static 64, then ranges claimed from `next`.
+    // While at least a grid's worth of rays remains beyond the current range, the next 64-ray
+    // range is claimed one ahead of need (lane 0 holds its base), so a refill waits for no
+    // atomic; otherwise a refill claims exactly the rays it lacks, so the end of the batch
+    // stays balanced across waves.
+    uint32_t cur_next = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u + 64u;
+    uint32_t cur_end = cur_next;
+    uint32_t pend = 0u;
+    bool has_pend = false;  // wave-uniform
     for (;;) {
-        if (!exhausted && !first) {
+        if (kRanged && !exhausted && !first) {
+            const bool need = id == kNone;
+            const unsigned long long mask = __ballot(need);
+            const uint32_t cnt_need = uint32_t(__popcll(mask));
+            if (cnt_need > 0) {
+                const uint32_t rank = uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+                const uint32_t avail = cur_end - cur_next;
+                uint32_t k = cur_next + rank;
+                if (cnt_need > avail) {
+                    uint32_t base = 0u;
+                    if (has_pend) {
+                        base = __shfl(pend, 0, 64);
+                        cur_next = base + (cnt_need - avail);
+                        cur_end = base + 64u;
+                        has_pend = false;
+                    } else {
+                        if (lane == 0u) base = static_total + atomicAdd(next, cnt_need - avail);
+                        base = __shfl(base, 0, 64);
+                        cur_next = cur_end = base + (cnt_need - avail);
+                    }
+                    if (rank >= avail) k = base + (rank - avail);
+                } else {
+                    cur_next += cnt_need;
+                }
+                if (need && k < n) start(k);
+                if (cur_next >= n) {
+                    exhausted = true;
+                } else if (!has_pend && cur_end + static_total < n) {
+                    if (lane == 0u) pend = static_total + atomicAdd(next, 64u);
+                    has_pend = true;
+                }
+            }
+        }
+        if (!kRanged && !exhausted && !first) {
             const bool need = id == kNone;
             const unsigned long long mask = __ballot(need);
             const uint32_t cnt_need = uint32_t(__popcll(mask));
@@ -479,7 +522,7 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
                 id = kNone;
             }
             const uint32_t idle = uint32_t(__popcll(__ballot(id == kNone)));
-            if (idle == 64u || (!exhausted && idle >= uint32_t(kRefillIdle))) break;
+            if (idle == 64u || (!exhausted && idle >= uint32_t(kRanged ? kRefillIdleRanged : kRefillIdle))) break;
         }
     }
 }
