@@ -163,7 +163,8 @@ typedef struct lmgpu_pt_params {
     int rr_depth;             /* -1 = no Russian roulette (pathtrace.cpp:125) */
     uint64_t seed;            /* per-sample Philox4x32-10 stream key */
     uint64_t sample_offset;   /* global index of the first sample (shards of a larger render) */
-    uint32_t wavefront_width; /* paths in flight per batch; 0 = auto */
+    uint32_t wavefront_width; /* paths in flight per batch; 0 = auto: the whole render, up to 2^26 paths
+                               * and a quarter of the free device memory */
     int film_precision;       /* LMGPU_FILM_* accumulation type on the device */
     /* Optional: a device buffer of num_samples float4 (LMGPU_PT) or 2 * num_samples float4
      * (LMGPU_PT_DIRECT, LMGPU_PT_MIS).  The render writes each sample's contribution (rgb)

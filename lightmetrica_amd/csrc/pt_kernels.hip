@@ -1059,6 +1059,22 @@ static void carve(void* base, uint32_t cap, PTBuffers& b) {
 
 static size_t pt_bytes(uint32_t cap) { return 2 * queue_bytes(cap) + size_t(cap) * (16 * 4 + 8) + 4096 + 64; }
 
+// Paths in flight per batch when the caller leaves it to the library: the whole render in one
+// batch, up to 2^26 paths (15.6 GB of queues) and a quarter of the device memory free for the
+// queues and the adjoint's per-sample totals.  Wide batches keep every segment's launch full:
+// with RR the late segments of a 2^21-path batch hold a few hundred thousand paths, fewer than
+// the persistent traversal grid has lanes.
+static uint32_t auto_width(const lmgpu_ctx* ctx, uint64_t num_samples) {
+    uint64_t cap = std::min<uint64_t>(num_samples, 1ull << 26);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const uint64_t per_path = pt_bytes(1) - 4096 - 64 + 32;
+        const uint64_t by_mem = (uint64_t(free_b) + ctx->pt_buf_bytes + ctx->adj_buf_bytes) / 4 / per_path;
+        cap = std::min<uint64_t>(cap, std::max<uint64_t>(by_mem, 1ull << 20));
+    }
+    return uint32_t(cap);
+}
+
 // (Re)allocates a context-owned device buffer to at least `bytes`.
 static lmgpu_status ensure(lmgpu_ctx* ctx, void** buf, size_t* have, size_t bytes, const char* what) {
     if (bytes <= *have) return LMGPU_OK;
@@ -1105,7 +1121,7 @@ struct Pipeline {
     bool is_pt;
 
     lmgpu_status setup(const lmgpu_pt_params* p) {
-        cap = p->wavefront_width ? p->wavefront_width : (1u << 21);
+        cap = p->wavefront_width ? p->wavefront_width : auto_width(ctx, p->num_samples);
         cap = uint32_t(std::min<uint64_t>(cap, p->num_samples));
         cap = std::max<uint32_t>(cap, 64u);
         lmgpu_status st = ensure(ctx, &ctx->pt_buf, &ctx->pt_buf_bytes, pt_bytes(cap), "path queues");
