@@ -131,6 +131,8 @@ struct lmgpu_ctx {
     size_t film_out_staging_bytes = 0;
     void* adj_buf = nullptr;  // adjoint: per-sample Q, gradients, staged dL/dI
     size_t adj_buf_bytes = 0;
+    void* splat_buf = nullptr;  // NEE renders: film replicas for the camera vertex's light connections
+    size_t splat_buf_bytes = 0;
 
     // Timing
     bool timing = false;

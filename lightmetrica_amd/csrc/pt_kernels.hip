@@ -71,6 +71,9 @@ struct PTConsts {
     double* grad_mat;    // adjoint: num_materials * 3
     double* grad_light;  // adjoint: num_lights * 3
     float4* record;      // pt render: per batch sample (radiance, pixel bits), or nullptr
+    void* splat;         // NEE render: splat_k film replicas of splat_vals values (FP32 for the FP32 film, else FP64)
+    uint32_t splat_k;
+    size_t splat_vals;
     uint64_t seed;
     uint64_t base;  // global index of the batch's first sample
     uint32_t batch;  // samples in this batch
@@ -163,6 +166,27 @@ __device__ __forceinline__ void film_add(const PTConsts& c, uint32_t pixel, V3 v
         unsafeAtomicAdd(f + 0, v.x);
         unsafeAtomicAdd(f + 1, v.y);
         unsafeAtomicAdd(f + 2, v.z);
+    }
+}
+
+// The camera vertex's light connections (pt.direct / pt.mis) all land on the pixels that
+// show the lights, so their film atomics pile onto a few cache lines and serialise there.
+// Block b adds them to film replica b mod splat_k instead (a multiple of the 8 XCDs, so each
+// replica is written from one XCD), and k_splat_fold sums the replicas into the film after
+// the last batch.
+__device__ __forceinline__ void splat_add(const PTConsts& c, uint32_t pixel, V3 v) {
+    if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return;
+    const size_t off = size_t(blockIdx.x % c.splat_k) * c.splat_vals + 3ull * pixel;
+    if (c.film_mode == LMGPU_FILM_FP32) {
+        float* f = static_cast<float*>(c.splat) + off;
+        unsafeAtomicAdd(f + 0, v.x);
+        unsafeAtomicAdd(f + 1, v.y);
+        unsafeAtomicAdd(f + 2, v.z);
+    } else {
+        double* f = static_cast<double*>(c.splat) + off;
+        unsafeAtomicAdd(f + 0, double(v.x));
+        unsafeAtomicAdd(f + 1, double(v.y));
+        unsafeAtomicAdd(f + 2, double(v.z));
     }
 }
 
@@ -861,7 +885,8 @@ struct ShadowSource {
     }
 };
 
-template <int kMode>
+// kSplat: the camera vertex's connections, added to the film replicas (splat_add).
+template <int kMode, bool kSplat>
 struct ShadowSink {
     PTConsts c;
     ShadowQueue sq;
@@ -875,18 +900,19 @@ struct ShadowSink {
         const V3 Le = light_Le(c, __float_as_int(v.w));
         const uint32_t pixel = __float_as_uint(sq.d[i].w);
         const V3 x = v3(v.x, v.y, v.z) * Le;
-        film_add(c, pixel, x);
+        if (kSplat) splat_add(c, pixel, x);
+        else film_add(c, pixel, x);
         if (c.record) record_add(c, sq.sid[i], pixel, x);
     }
 };
 
-template <int kMode>
+template <int kMode, bool kSplat = false>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue sq, const uint32_t* __restrict__ count,
                                                         uint32_t* next) {
     __shared__ TraceShared<false> sm;
     const uint32_t n = *count;
     ShadowSource src{sq};
-    ShadowSink<kMode> sink{c, sq};
+    ShadowSink<kMode, kSplat> sink{c, sq};
     TraceCounters cnt{0, 0};
     trace_persistent<true, false, false>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
 }
@@ -993,6 +1019,34 @@ __global__ void k_resolve_comp(const float* __restrict__ acc, const float* __res
     if (i < n) out[i] = Out((double(acc[i]) + double(err[i])) * weight);
 }
 
+// The film replicas of splat_add summed into the film accumulator (value i of every
+// replica), once, after the last batch: stream order makes plain read-modify-writes safe.
+// The compensated film takes sum + error + replicas in FP64, stored back as an FP32 pair.
+__global__ void k_splat_fold(PTConsts c, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (c.film_mode == LMGPU_FILM_FP32) {
+        const float* r = static_cast<const float*>(c.splat) + i;
+        float s = 0.f;
+        for (uint32_t k = 0; k < c.splat_k; k++) s += r[k * c.splat_vals];
+        static_cast<float*>(c.film)[i] += s;
+        return;
+    }
+    const double* r = static_cast<const double*>(c.splat) + i;
+    double s = 0.0;
+    for (uint32_t k = 0; k < c.splat_k; k++) s += r[k * c.splat_vals];
+    if (c.film_mode == LMGPU_FILM_FP64) {
+        static_cast<double*>(c.film)[i] += s;
+    } else {
+        float* f = static_cast<float*>(c.film) + i;
+        float* e = c.film_comp + i;
+        const double t = double(*f) + double(*e) + s;
+        const float hi = float(t);
+        *f = hi;
+        *e = float(t - double(hi));
+    }
+}
+
 // sum over samples of |Q.x| + |Q.y| + |Q.z| (the replay check of the adjoint).
 __global__ __launch_bounds__(256) void k_abs_sum(const double* __restrict__ q, uint32_t n, double* out) {
     __shared__ double part[256 / 64];
@@ -1035,6 +1089,11 @@ __global__ void k_scale_f64(double* p, uint32_t n, double w) {
 
 // ---------------------------------------------------------------------------------
 // Host driver
+
+// Film replicas of the NEE renders' camera connections: at most kSplatMax (a multiple of
+// the 8 XCDs), within kSplatBudget bytes.
+constexpr size_t kSplatMax = 32;
+constexpr size_t kSplatBudget = size_t(1) << 30;
 
 static size_t queue_bytes(uint32_t cap) { return size_t(cap) * 16 * 5; }
 
@@ -1177,6 +1236,13 @@ struct Pipeline {
         if (e != hipSuccess) return hip_fail(ctx, e, "k_gen");
         auto shadow = [&](int k) -> lmgpu_status {
             hipError_t e2 = timed(ctx, kKShadow, [&]() {
+                if constexpr (kMode == kRender) {
+                    if (k == 0 && c.splat) {
+                        hipLaunchKernelGGL((k_shadow<kRender, true>), grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
+                                           b.counters + kMaxSegments + k, b.counters + 3 * kMaxSegments + k);
+                        return;
+                    }
+                }
                 hipLaunchKernelGGL(k_shadow<kMode>, grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
                                    b.counters + kMaxSegments + k, b.counters + 3 * kMaxSegments + k);
             });
@@ -1304,6 +1370,20 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
     }
     pl.c.film = ctx->film_acc;
     pl.c.film_comp = comp ? reinterpret_cast<float*>(static_cast<char*>(ctx->film_acc) + acc_one) : nullptr;
+    // NEE renderers: film replicas for the camera vertex's light connections (splat_add),
+    // as many as fit kSplatBudget, at most kSplatMax.
+    size_t splat_bytes = 0;
+    if (p->renderer != LMGPU_PT) {
+        const size_t elem = p->film_precision == LMGPU_FILM_FP32 ? 4 : 8;
+        const size_t one = (nval * elem + 255) / 256 * 256;
+        const uint32_t k = uint32_t(std::min<size_t>(kSplatMax, std::max<size_t>(1, kSplatBudget / one)));
+        splat_bytes = size_t(k) * one;
+        st = ensure(ctx, &ctx->splat_buf, &ctx->splat_buf_bytes, splat_bytes, "film replicas");
+        if (st != LMGPU_OK) return st;
+        pl.c.splat = ctx->splat_buf;
+        pl.c.splat_k = k;
+        pl.c.splat_vals = one / elem;
+    }
 
     hipStream_t s = ctx->stream;
     hipEvent_t r0, r1;
@@ -1312,6 +1392,9 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
     (void)hipEventRecord(r0, s);
     hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((acc_bytes / 16 + 255) / 256)), dim3(256), 0, s,
                        static_cast<uint4*>(ctx->film_acc), acc_bytes / 16);
+    if (splat_bytes)
+        hipLaunchKernelGGL(k_zero_bytes, dim3(uint32_t((splat_bytes / 16 + 255) / 256)), dim3(256), 0, s,
+                           static_cast<uint4*>(ctx->splat_buf), splat_bytes / 16);
     hipLaunchKernelGGL(k_zero_u32, dim3(5), dim3(256), 0, s, pl.b.counters, (4096u + 64u) / 4u);
     const uint32_t rec_stride = p->renderer == LMGPU_PT ? 1u : 2u;  // float4 per sample in the record
     for (uint64_t base = 0; base < p->num_samples && st == LMGPU_OK; base += pl.cap) {
@@ -1321,6 +1404,7 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
     if (st == LMGPU_OK) {
         // Resolve: sums * W*H/N into the output film.
         const dim3 rg(uint32_t((nval + 255) / 256));
+        if (splat_bytes) hipLaunchKernelGGL(k_splat_fold, rg, dim3(256), 0, s, pl.c, nval);
         if (comp) {
             const double w = double(npix) / double(p->num_samples);
             const float* acc = static_cast<const float*>(ctx->film_acc);

@@ -126,6 +126,23 @@ def test_fp32_film_matches_fp64_film(gpu):
     np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6)
 
 
+@pytest.mark.parametrize("renderer", ["pt.direct", "pt.mis"])
+def test_nee_film_modes_agree(gpu, renderer):
+    """The camera vertex's light connections go through film replicas that are folded into
+    the film after the last batch; every film mode gives the same image (the room's light
+    covers a few hundred pixels that take a connection from every sample)."""
+    sc = scenes.room_scene(64, 64, 32)
+    gpu.upload(sc)
+    gpu.build_bvh()
+    ref, _ = gpu.render(renderer, film_precision="fp64")
+    f32, _ = gpu.render(renderer, film_precision="fp32")
+    comp, _ = gpu.render(renderer, film_precision="fp32c")
+    three, _ = gpu.render(renderer, film_precision="fp64", wavefront_width=40000)  # 4 batches
+    np.testing.assert_allclose(f32, ref, rtol=2e-5, atol=1e-6)
+    np.testing.assert_allclose(comp, ref, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(three, ref, rtol=1e-9, atol=1e-12)
+
+
 def test_compensated_fp32_film(gpu):
     """Many samples per pixel (a 4 x 4 film, 2^22 samples: 2^18 per pixel): the FP32 sums
     drift from the FP64 sums of the same contributions (small terms are rounded away once the
