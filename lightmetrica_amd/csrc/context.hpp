@@ -111,6 +111,7 @@ struct lmgpu_ctx {
     uint32_t num_lights = 0, num_materials = 0;
     std::vector<lmgpu::MaterialDev> host_mats;  // copies for lmgpu_scene_set_params
     std::vector<lmgpu::LightDev> host_lights;
+    std::vector<lmgpu::LightTri> host_light_tris;  // the area lights' triangles (camera-connection pixels)
     lmgpu::CameraDev cam{};
     lmgpu::EnvDev env{};
 

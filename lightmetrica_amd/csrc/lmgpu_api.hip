@@ -373,6 +373,7 @@ lmgpu_status lmgpu_scene_upload(lmgpu_ctx* ctx, const lmgpu_scene_desc* d) {
     if (!ctx->has_camera) return fail(ctx, LMGPU_ERR_INVALID_ARGUMENT, "lmgpu_scene_upload: shading scenes need a camera");
     ctx->host_mats = std::move(mats);
     ctx->host_lights = std::move(lights);
+    ctx->host_light_tris = std::move(ltris);
     ctx->has_shading = true;
     return LMGPU_OK;
 }

@@ -33,6 +33,7 @@
 #include "traverse.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 
 // Path state arithmetic of this file is not contracted into FMAs, so the render and both
@@ -74,6 +75,7 @@ struct PTConsts {
     void* splat;         // NEE render: splat_k film replicas of splat_vals values (FP32 for the FP32 film, else FP64)
     uint32_t splat_k;
     size_t splat_vals;
+    uint32_t splat_x0, splat_y0, splat_w, splat_h;  // the film rectangle the replicas cover
     uint64_t seed;
     uint64_t base;  // global index of the batch's first sample
     uint32_t batch;  // samples in this batch
@@ -171,12 +173,19 @@ __device__ __forceinline__ void film_add(const PTConsts& c, uint32_t pixel, V3 v
 
 // The camera vertex's light connections (pt.direct / pt.mis) all land on the pixels that
 // show the lights, so their film atomics pile onto a few cache lines and serialise there.
-// Block b adds them to film replica b mod splat_k instead (a multiple of the 8 XCDs, so each
-// replica is written from one XCD), and k_splat_fold sums the replicas into the film after
-// the last batch.
+// Block b adds them to replica b mod splat_k (a multiple of the 8 XCDs when at least 8, so
+// each replica is written from one XCD) of the film rectangle that the lights project to,
+// and k_splat_fold sums the replicas into the film after the last batch.  A pixel outside
+// the rectangle (never, up to rounding) goes to the film directly.
 __device__ __forceinline__ void splat_add(const PTConsts& c, uint32_t pixel, V3 v) {
     if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return;
-    const size_t off = size_t(blockIdx.x % c.splat_k) * c.splat_vals + 3ull * pixel;
+    const uint32_t w = uint32_t(c.cam.width);
+    const uint32_t x = pixel % w - c.splat_x0, y = pixel / w - c.splat_y0;  // wraps below the rectangle
+    if (x >= c.splat_w || y >= c.splat_h) {
+        film_add(c, pixel, v);
+        return;
+    }
+    const size_t off = size_t(blockIdx.x % c.splat_k) * c.splat_vals + 3ull * (y * c.splat_w + x);
     if (c.film_mode == LMGPU_FILM_FP32) {
         float* f = static_cast<float*>(c.splat) + off;
         unsafeAtomicAdd(f + 0, v.x);
@@ -1020,26 +1029,31 @@ __global__ void k_resolve_comp(const float* __restrict__ acc, const float* __res
 }
 
 // The film replicas of splat_add summed into the film accumulator (value i of every
-// replica), once, after the last batch: stream order makes plain read-modify-writes safe.
-// The compensated film takes sum + error + replicas in FP64, stored back as an FP32 pair.
+// replica, i < 3 * splat_w * splat_h), once, after the last batch: stream order makes plain
+// read-modify-writes safe.  The compensated film takes sum + error + replicas in FP64,
+// stored back as an FP32 pair.
 __global__ void k_splat_fold(PTConsts c, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t local = uint32_t(i / 3), ch = uint32_t(i % 3);
+    const size_t j = 3ull * ((c.splat_y0 + local / c.splat_w) * size_t(c.cam.width) + c.splat_x0 + local % c.splat_w) + ch;
     if (c.film_mode == LMGPU_FILM_FP32) {
         const float* r = static_cast<const float*>(c.splat) + i;
         float s = 0.f;
+#pragma unroll 8
         for (uint32_t k = 0; k < c.splat_k; k++) s += r[k * c.splat_vals];
-        static_cast<float*>(c.film)[i] += s;
+        static_cast<float*>(c.film)[j] += s;
         return;
     }
     const double* r = static_cast<const double*>(c.splat) + i;
     double s = 0.0;
+#pragma unroll 8
     for (uint32_t k = 0; k < c.splat_k; k++) s += r[k * c.splat_vals];
     if (c.film_mode == LMGPU_FILM_FP64) {
-        static_cast<double*>(c.film)[i] += s;
+        static_cast<double*>(c.film)[j] += s;
     } else {
-        float* f = static_cast<float*>(c.film) + i;
-        float* e = c.film_comp + i;
+        float* f = static_cast<float*>(c.film) + j;
+        float* e = c.film_comp + j;
         const double t = double(*f) + double(*e) + s;
         const float hi = float(t);
         *f = hi;
@@ -1090,10 +1104,52 @@ __global__ void k_scale_f64(double* p, uint32_t n, double w) {
 // ---------------------------------------------------------------------------------
 // Host driver
 
-// Film replicas of the NEE renders' camera connections: at most kSplatMax (a multiple of
-// the 8 XCDs), within kSplatBudget bytes.
+// Film replicas of the NEE renders' camera connections (splat_add).
 constexpr size_t kSplatMax = 32;
-constexpr size_t kSplatBudget = size_t(1) << 30;
+constexpr size_t kSplatMinBytes = size_t(32) << 20;
+constexpr size_t kSplatMaxBytes = size_t(1) << 30;
+
+// The film rectangle {x0, y0, x1, y1} (inclusive) that the camera vertex's light connections
+// can reach: the raster bounding box of the area lights' triangles, widened by a pixel on
+// each side for rounding (a point of a triangle in front of the camera projects inside the
+// projected triangle); the whole film with an environment light, or when a light vertex is
+// not in front of the camera.
+static void connection_rect(const lmgpu_ctx* ctx, uint32_t r[4]) {
+    const CameraDev& cam = ctx->cam;
+    r[0] = 0;
+    r[1] = 0;
+    r[2] = uint32_t(cam.width - 1);
+    r[3] = uint32_t(cam.height - 1);
+    if (ctx->env.index >= 0 || ctx->host_light_tris.empty()) return;
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    auto mul = [](const M4& m, const double x[4], double o[4]) {
+        for (int i = 0; i < 4; i++)
+            o[i] = m.m[i] * x[0] + m.m[4 + i] * x[1] + m.m[8 + i] * x[2] + m.m[12 + i] * x[3];
+    };
+    for (const LightTri& t : ctx->host_light_tris) {
+        for (const float* p : {t.p1, t.p2, t.p3}) {
+            const double q[4] = {p[0], p[1], p[2], 1.0};
+            double v[4], n[4];
+            mul(cam.view, q, v);
+            if (!(v[2] < 0.0)) return;  // the camera looks down -z
+            mul(cam.proj, v, n);
+            if (!(n[3] > 0.0)) return;
+            const double rx = (n[0] / n[3] + 1.0) / 2.0 * cam.width, ry = (n[1] / n[3] + 1.0) / 2.0 * cam.height;
+            if (!std::isfinite(rx) || !std::isfinite(ry)) return;
+            lo[0] = std::min(lo[0], rx);
+            hi[0] = std::max(hi[0], rx);
+            lo[1] = std::min(lo[1], ry);
+            hi[1] = std::max(hi[1], ry);
+        }
+    }
+    const double size[2] = {double(cam.width), double(cam.height)};
+    for (int a = 0; a < 2; a++) {
+        const double l = std::min(std::max(std::floor(lo[a]) - 1.0, 0.0), size[a] - 1.0);
+        const double h = std::min(std::max(std::floor(hi[a]) + 1.0, 0.0), size[a] - 1.0);
+        r[a] = uint32_t(l);
+        r[2 + a] = uint32_t(h);
+    }
+}
 
 static size_t queue_bytes(uint32_t cap) { return size_t(cap) * 16 * 5; }
 
@@ -1370,18 +1426,28 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
     }
     pl.c.film = ctx->film_acc;
     pl.c.film_comp = comp ? reinterpret_cast<float*>(static_cast<char*>(ctx->film_acc) + acc_one) : nullptr;
-    // NEE renderers: film replicas for the camera vertex's light connections (splat_add),
-    // as many as fit kSplatBudget, at most kSplatMax.
-    size_t splat_bytes = 0;
+    // NEE renderers: replicas of the film rectangle the lights project to, for the camera
+    // vertex's light connections (splat_add): at most kSplatMax, within 8 bytes per sample
+    // clamped to [kSplatMinBytes, kSplatMaxBytes], a multiple of 8 from 8 on.
+    size_t splat_bytes = 0, splat_vals = 0;
     if (p->renderer != LMGPU_PT) {
+        uint32_t r[4];
+        connection_rect(ctx, r);
+        pl.c.splat_x0 = r[0];
+        pl.c.splat_y0 = r[1];
+        pl.c.splat_w = r[2] - r[0] + 1;
+        pl.c.splat_h = r[3] - r[1] + 1;
+        splat_vals = size_t(pl.c.splat_w) * pl.c.splat_h * 3;
         const size_t elem = p->film_precision == LMGPU_FILM_FP32 ? 4 : 8;
-        const size_t one = (nval * elem + 255) / 256 * 256;
-        const uint32_t k = uint32_t(std::min<size_t>(kSplatMax, std::max<size_t>(1, kSplatBudget / one)));
-        splat_bytes = size_t(k) * one;
+        const size_t one = (splat_vals * elem + 255) / 256 * 256;
+        const size_t budget = std::min(kSplatMaxBytes, std::max(kSplatMinBytes, size_t(p->num_samples) * 8));
+        size_t k = std::min(kSplatMax, std::max<size_t>(1, budget / one));
+        if (k >= 8) k &= ~size_t(7);
+        splat_bytes = k * one;
         st = ensure(ctx, &ctx->splat_buf, &ctx->splat_buf_bytes, splat_bytes, "film replicas");
         if (st != LMGPU_OK) return st;
         pl.c.splat = ctx->splat_buf;
-        pl.c.splat_k = k;
+        pl.c.splat_k = uint32_t(k);
         pl.c.splat_vals = one / elem;
     }
 
@@ -1404,7 +1470,8 @@ lmgpu_status run_pt(lmgpu_ctx* ctx, const lmgpu_pt_params* p, lmgpu_film* film, 
     if (st == LMGPU_OK) {
         // Resolve: sums * W*H/N into the output film.
         const dim3 rg(uint32_t((nval + 255) / 256));
-        if (splat_bytes) hipLaunchKernelGGL(k_splat_fold, rg, dim3(256), 0, s, pl.c, nval);
+        if (splat_bytes)
+            hipLaunchKernelGGL(k_splat_fold, dim3(uint32_t((splat_vals + 255) / 256)), dim3(256), 0, s, pl.c, splat_vals);
         if (comp) {
             const double w = double(npix) / double(p->num_samples);
             const float* acc = static_cast<const float*>(ctx->film_acc);
