@@ -420,8 +420,13 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t static_total = waves * 64u;
     {
-        const uint32_t k = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u + lane;
-        if (k < n) start(k);
+        // A batch of at most 64 rays per wave is spread evenly over all waves, in chunks of
+        // ceil(n / waves) consecutive rays: a wave runs as long as the slowest of its rays
+        // (and the four-cycle issue of each instruction does not shrink with the lanes in
+        // use), so fewer rays per wave end a small batch -- a late bounce -- sooner.
+        const uint32_t chunk = static_total >= n ? (n + waves - 1u) / waves : 64u;
+        const uint32_t k = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * chunk + lane;
+        if (lane < chunk && k < n) start(k);
     }
     bool exhausted = static_total >= n;  // wave-uniform
     bool first = true;
