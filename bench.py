@@ -78,6 +78,8 @@ def traffic_from_profile():
 
 
 def run_traversal(args, torch, Device, scenes):
+    import numpy as np
+
     t0 = time.time()
     sc = scenes.traversal_scene(args.triangles)
     rays = scenes.traversal_rays(args.rays)
@@ -127,6 +129,16 @@ def run_traversal(args, torch, Device, scenes):
     alg_bytes = n * (RAY_BYTES + HIT_BYTES) + visits * NODE_BYTES + tests * TRI_BYTES
     prim = out[3].cpu().numpy()
     hit_rate = float((prim >= 0).mean())
+
+    # The same rays from pageable host arrays through lmgpu_trace_closest, the form in which a
+    # Scene::Intersect caller hands rays over: staging copies over PCIe in and hits out
+    # included (reported beside the value, never as it).
+    host_cols = dev.ray_columns(rays)
+    dev.trace_closest_columns([c[: 1 << 20] for c in host_cols], refine_fp64=False)  # staging buffers
+    t0 = time.perf_counter()
+    host_hits = dev.trace_closest_columns(host_cols, refine_fp64=False)
+    host_s = time.perf_counter() - t0
+    host_same = bool(np.array_equal(host_hits[3], prim))
     dev.close()
     res = {
         "ms_per_step": wall / args.steps * 1e3,
@@ -137,6 +149,10 @@ def run_traversal(args, torch, Device, scenes):
         "tri_tests_per_ray": tests / n,
         "hit_rate": hit_rate,
         "bvh": info,
+        "host_buffers": {"Mrays_per_s": round(n / host_s / 1e6, 1), "seconds": round(host_s, 4),
+                         "same_hits_as_device_buffers": host_same,
+                         "note": "lmgpu_trace_closest on pageable host arrays (32 B per ray in, 16 B per hit "
+                                 "out over PCIe), one call over the same rays"},
     }
     return res, sc, rays
 
@@ -570,6 +586,7 @@ def main():
             "bvh_depth": res["bvh"].max_depth,
             "bvh_build_s": round(res["bvh"].build_seconds, 2),
         },
+        "host_buffers": res["host_buffers"],
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),

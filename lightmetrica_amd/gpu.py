@@ -149,8 +149,18 @@ class Device:
 
     def trace_closest(self, rays: Rays, refine_fp64: bool = True):
         """Closest hits of host rays: (t, b0, b1, prim) numpy arrays; prim = -1 for a miss."""
-        n = len(rays)
-        soa, keep = self._host_rays(rays)
+        return self.trace_closest_columns(self.ray_columns(rays), refine_fp64)
+
+    @staticmethod
+    def ray_columns(rays: Rays) -> list:
+        """The rays as the eight contiguous float32 host columns of lmgpu_rays_soa."""
+        return Device._host_rays(rays)[1]
+
+    def trace_closest_columns(self, cols: list, refine_fp64: bool = True):
+        """lmgpu_trace_closest on host columns (ray_columns): the library stages them to the
+        device, traces and copies the hits back."""
+        n = int(cols[0].shape[0])
+        soa = N.RaysSoA(*[c.ctypes.data for c in cols], 0)
         t = np.empty(n, np.float32)
         b0 = np.empty(n, np.float32)
         b1 = np.empty(n, np.float32)

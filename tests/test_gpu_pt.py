@@ -67,6 +67,32 @@ def test_env_scene_same_streams(gpu, oracle, renderer):
     same_stream_check(g, o, f"env/{renderer}")
 
 
+def room_camera_under_light(res=48, spp=32):
+    """The C1 room seen from just under its light, looking up past it: the light's far
+    vertices lie behind the camera plane while its near half is in view, so the camera
+    vertex's connections cannot be bounded by the light's raster rectangle (the whole film
+    takes them)."""
+    sc = scenes.room_scene(res, res, spp)
+    eye = np.array([0.5, 0.95, 0.5])
+    view = scenes.look_at(tuple(eye), tuple(eye + np.array([0.0, 0.3, -0.954])), (0, 1, 0)).astype(np.float32)
+    sc.camera = scenes.Camera(view=view, fovy=60.0, width=res, height=res)
+    sc.name = "room-under-light"
+    return sc
+
+
+@pytest.mark.parametrize("renderer", ["pt.direct", "pt.mis"])
+def test_light_across_camera_plane_same_streams(gpu, oracle, renderer):
+    sc = room_camera_under_light()
+    gpu.upload(sc)
+    gpu.build_bvh()
+    g, _ = gpu.render(renderer, film_precision="fp64")
+    o, _ = oracle.Oracle(sc, fp64=True, accel="bvh").render(renderer)
+    assert o.max() > 0.0
+    same_stream_check(g, o, f"room-under-light/{renderer}")
+    f32, _ = gpu.render(renderer, film_precision="fp32")
+    np.testing.assert_allclose(f32, g, rtol=2e-5, atol=1e-6)
+
+
 def random_scene(case: int, res=32, spp=16):
     """A random open scene: a ground quad, 1-6 rotated boxes with random diffuse / mirror /
     diffuse-mirror materials, 1-3 area lights of random emission, a constant environment
