@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I", str(ROOT / "include")]
 
-LIB_SOURCES = ["lmgpu_api.hip", "trace_kernels.hip", "pt_kernels.hip", "bvh_build.cpp"]
+LIB_SOURCES = ["lmgpu_api.hip", "trace_kernels.hip", "pt_kernels.hip", "ray_order.hip", "bvh_build.cpp"]
 LIB = PKG / "liblmgpu.so"
 
 

@@ -134,6 +134,8 @@ struct lmgpu_ctx {
     size_t adj_buf_bytes = 0;
     void* splat_buf = nullptr;  // NEE renders: film replicas for the camera vertex's light connections
     size_t splat_buf_bytes = 0;
+    void* order_buf = nullptr;  // pt on large scenes: tile keys, sample order and the sort's scratch
+    size_t order_buf_bytes = 0;
 
     // Timing
     bool timing = false;

@@ -185,6 +185,7 @@ lmgpu_status lmgpu_destroy(lmgpu_ctx* ctx) {
     dev_free(ctx->film_out_staging);
     dev_free(ctx->adj_buf);
     dev_free(ctx->splat_buf);
+    dev_free(ctx->order_buf);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
