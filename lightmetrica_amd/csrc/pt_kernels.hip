@@ -618,15 +618,19 @@ __device__ __forceinline__ uint32_t morton2(uint32_t x, uint32_t y) {
     return spread(x) | (spread(y) << 1);
 }
 
-// Sort keys of pt's camera stage on large scenes: the Z-order index of the 8x8-pixel tile
-// that each batch sample's raster position (the first two numbers of its stream,
-// pathtrace.cpp:167-173) falls in.  Tracing the camera rays tile by tile lets a wave's
-// lanes share BVH nodes in the caches; the samples and their streams are unchanged.
-__global__ void k_tile_keys(PTConsts c, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// Sort keys of the camera stage on large scenes: the Z-order index of the 8x8-pixel tile
+// that each batch sample's raster position falls in -- numbers `skip` and skip + 1 of its
+// stream: 0 for pt (pathtrace.cpp:167-173); for pt.direct / pt.mis the camera position, the
+// light selection and sample of the camera vertex's connection and, when rr_depth <= 1, the
+// roulette come first (pathtrace.direct.cpp:175-262).  Tracing the camera rays tile by tile
+// lets a wave's lanes share BVH nodes in the caches; the samples and their streams are
+// unchanged.
+__global__ void k_tile_keys(PTConsts c, int skip, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.batch) return;
     SampleStream rng;
     rng.init(c.seed, c.base + i, 0);
+    for (int k = 0; k < skip; k++) rng.next();
     const float r0 = rng.next(), r1 = rng.next();
     const uint32_t p = pixel_of(c.cam, r0, r1), w = uint32_t(c.cam.width);
     keys[i] = morton2((p % w) >> 3, (p / w) >> 3);
@@ -1133,7 +1137,7 @@ __global__ void k_scale_f64(double* p, uint32_t n, double w) {
 // ---------------------------------------------------------------------------------
 // Host driver
 
-// pt orders its camera rays by screen tile when the BVH is at least as large as one XCD's
+// The camera stage orders its rays by screen tile when the BVH is at least as large as one XCD's
 // L2 (4 MB); a smaller one stays in the caches whatever the order, and the sort would only
 // add its own time (room: 0.90 -> 1.01 ms; 7.4 MB gradient scene: 11.35 -> 10.72 ms).
 constexpr uint64_t kOrderMinBvhBytes = uint64_t(4) << 20;
@@ -1276,6 +1280,7 @@ struct Pipeline {
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     int key_bits = 0;
+    int key_skip = 0;
 
     lmgpu_status setup(const lmgpu_pt_params* p) {
         cap = p->wavefront_width ? p->wavefront_width : auto_width(ctx, p->num_samples);
@@ -1310,10 +1315,12 @@ struct Pipeline {
         // Segments traced per path: at most max_path_vertices - 1 (pathtrace.cpp:196-199).
         segments = p->max_path_vertices - 1;
         is_pt = p->renderer == LMGPU_PT;
-        // pt on a BVH larger than the caches: camera rays in screen-tile order (k_tile_keys).
+        // A BVH larger than the caches: camera rays in screen-tile order (k_tile_keys), whose
+        // raster numbers follow key_skip others in each stream.
         c.order = nullptr;
+        key_skip = is_pt ? 0 : 4 + (p->rr_depth != -1 && p->rr_depth <= 1 ? 1 : 0);
         const uint64_t bvh_bytes = uint64_t(ctx->nodes.count) * sizeof(Node4) + uint64_t(ctx->tris.count) * sizeof(TriRecord);
-        if (is_pt && bvh_bytes >= kOrderMinBvhBytes) {
+        if (bvh_bytes >= kOrderMinBvhBytes) {
             const uint32_t tiles = uint32_t(std::max((ctx->cam.width + 7) / 8, (ctx->cam.height + 7) / 8));
             key_bits = 2;
             while ((1u << (key_bits / 2)) < tiles) key_bits += 2;
@@ -1352,7 +1359,7 @@ struct Pipeline {
         hipError_t sort_e = hipSuccess;
         hipError_t e = timed(ctx, kKGen, [&]() {
             if (c.order) {
-                hipLaunchKernelGGL(k_tile_keys, grid_b, dim3(kPTBlock), 0, s, c, keys_in, vals_in);
+                hipLaunchKernelGGL(k_tile_keys, grid_b, dim3(kPTBlock), 0, s, c, key_skip, keys_in, vals_in);
                 size_t tb = sort_temp_bytes;
                 sort_e = sort_tile_keys(sort_temp, tb, keys_in, keys_out, vals_in, vals_out, nb, key_bits, s);
             }
