@@ -744,9 +744,10 @@ __device__ __forceinline__ void grad_material(GradCache& g, const PTConsts& c, u
 }
 
 // pt: pathtrace.cpp:203-262 for one segment.
-// The shade kernels walk the queue in tiles of kPTBlock paths, tile t by block t mod grid:
-// the render launches one block per tile, the gradient pass a device-filling grid so that
-// each block's gradient table collects many paths before it is flushed.
+// The shade kernels walk the queue in tiles of kPTBlock paths, tile t by block t mod grid,
+// over a device-filling grid (16 blocks per CU; 4 for the gradient pass, so that each
+// block's gradient table collects many paths before it is flushed): a late segment's
+// nearly empty queue then costs a launch of that grid, not of one block per queue slot.
 template <int kMode>
 __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                        const float4* __restrict__ hits, Queue out,
@@ -811,12 +812,8 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
         if (c.record) c.record[meta.x] = make_float4(L.x, L.y, L.z, __uint_as_float(meta.w));
     }
     };
-    if (kMode == kAdjGrad) {
-        for (uint32_t tile = blockIdx.x; tile * kPTBlock < count; tile += gridDim.x) tile_body(tile);
-        gc_flush(gc, c);
-    } else if (blockIdx.x * kPTBlock < count) {
-        tile_body(blockIdx.x);
-    }
+    for (uint32_t tile = blockIdx.x; tile * kPTBlock < count; tile += gridDim.x) tile_body(tile);
+    if (kMode == kAdjGrad) gc_flush(gc, c);
 }
 
 // pt.direct / pt.mis: the hit half of one loop iteration (pathtrace.direct.cpp:304-336,
@@ -908,12 +905,8 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
         store_path(out, slot, s.p, kEps, wo, pdf, T, s.f.sn, meta.x, nf, rng.dim, meta.w);
     }
     };
-    if (kMode == kAdjGrad) {
-        for (uint32_t tile = blockIdx.x; tile * kPTBlock < count; tile += gridDim.x) tile_body(tile);
-        gc_flush(gc, c);
-    } else if (blockIdx.x * kPTBlock < count) {
-        tile_body(blockIdx.x);
-    }
+    for (uint32_t tile = blockIdx.x; tile * kPTBlock < count; tile += gridDim.x) tile_body(tile);
+    if (kMode == kAdjGrad) gc_flush(gc, c);
 }
 
 // NEE shadow rays: any hit over the triangles.  The render adds the unoccluded
@@ -1353,9 +1346,8 @@ struct Pipeline {
         const dim3 grid_b((nb + kPTBlock - 1) / kPTBlock), grid_c((cap + kPTBlock - 1) / kPTBlock);
         // Persistent traversal grids: the device fill, never more than the queue capacity needs.
         const dim3 grid_t(std::min<uint32_t>(grid_c.x, trace_grid));
-        // The gradient pass: 4 blocks per CU, each flushing its gradient table once.
-        const dim3 grid_s(kMode == kAdjGrad ? std::min<uint32_t>(grid_c.x, 4u * uint32_t(std::max(ctx->num_cus, 1)))
-                                            : grid_c.x);
+        // Shade grids: 16 blocks per CU; the gradient pass 4, each flushing its gradient table once.
+        const dim3 grid_s(std::min<uint32_t>(grid_c.x, (kMode == kAdjGrad ? 4u : 16u) * uint32_t(std::max(ctx->num_cus, 1))));
         hipError_t sort_e = hipSuccess;
         hipError_t e = timed(ctx, kKGen, [&]() {
             if (c.order) {
