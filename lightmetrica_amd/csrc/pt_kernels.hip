@@ -735,6 +735,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict_
     trace_persistent<false, false, false>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
+// Extension rays of a scene of at most kBruteMaxTris triangles: every triangle per ray.
+__global__ __launch_bounds__(kTraceBlock) void k_extend_brute(const TriRecord* __restrict__ tris, uint32_t ntris,
+                                                              Queue q, const uint32_t* __restrict__ count,
+                                                              float4* __restrict__ hits) {
+    const uint32_t n = *count;
+    QueueSource src{q};
+    HitSink sink{tris, hits};
+    trace_brute<false>(tris, ntris, n, src, sink);
+}
+
 // dL/dR of the vertex's material: Q / R per channel (the contributions still in Q all carry
 // R once).
 __device__ __forceinline__ void grad_material(GradCache& g, const PTConsts& c, uint32_t sid, uint32_t mat, V3 R) {
@@ -950,6 +960,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(PTConsts c, ShadowQueue 
     ShadowSink<kMode, kSplat> sink{c, sq};
     TraceCounters cnt{0, 0};
     trace_persistent<true, false, false>(c.nodes, c.tris, n, next, sm, src, sink, cnt);
+}
+
+// Shadow rays of a scene of at most kBruteMaxTris triangles.
+template <int kMode, bool kSplat = false>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow_brute(PTConsts c, uint32_t ntris, ShadowQueue sq,
+                                                              const uint32_t* __restrict__ count) {
+    const uint32_t n = *count;
+    ShadowSource src{sq};
+    ShadowSink<kMode, kSplat> sink{c, sq};
+    trace_brute<true>(c.tris, ntris, n, src, sink);
 }
 
 // Adjoint passes: the unoccluded shadow contributions into Q, and dL/dLe.
@@ -1268,6 +1288,7 @@ struct Pipeline {
     uint32_t trace_grid;
     int segments;
     bool is_pt;
+    bool brute;  // at most kBruteMaxTris triangles: no BVH descent (trace_brute)
     // pt camera-stage order (large scenes): keys and samples in, sorted out, sort scratch.
     uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
     void* sort_temp = nullptr;
@@ -1308,6 +1329,7 @@ struct Pipeline {
         // Segments traced per path: at most max_path_vertices - 1 (pathtrace.cpp:196-199).
         segments = p->max_path_vertices - 1;
         is_pt = p->renderer == LMGPU_PT;
+        brute = ctx->tris.count <= kBruteMaxTris;
         // A BVH larger than the caches: camera rays in screen-tile order (k_tile_keys), whose
         // raster numbers follow key_skip others in each stream.
         c.order = nullptr;
@@ -1346,6 +1368,9 @@ struct Pipeline {
         const dim3 grid_b((nb + kPTBlock - 1) / kPTBlock), grid_c((cap + kPTBlock - 1) / kPTBlock);
         // Persistent traversal grids: the device fill, never more than the queue capacity needs.
         const dim3 grid_t(std::min<uint32_t>(grid_c.x, trace_grid));
+        // Brute-force traversal grids: 16 blocks of kTraceBlock per CU, walking the queue.
+        const dim3 grid_r(std::min<uint32_t>((cap + kTraceBlock - 1) / kTraceBlock, 16u * uint32_t(std::max(ctx->num_cus, 1))));
+        const uint32_t ntris = uint32_t(ctx->tris.count);
         // Shade grids: 16 blocks per CU; the gradient pass 4, each flushing its gradient table once.
         const dim3 grid_s(std::min<uint32_t>(grid_c.x, (kMode == kAdjGrad ? 4u : 16u) * uint32_t(std::max(ctx->num_cus, 1))));
         hipError_t sort_e = hipSuccess;
@@ -1362,15 +1387,23 @@ struct Pipeline {
         if (e != hipSuccess) return hip_fail(ctx, e, "k_gen");
         auto shadow = [&](int k) -> lmgpu_status {
             hipError_t e2 = timed(ctx, kKShadow, [&]() {
+                const uint32_t* cnt = b.counters + kMaxSegments + k;
                 if constexpr (kMode == kRender) {
                     if (k == 0 && c.splat) {
-                        hipLaunchKernelGGL((k_shadow<kRender, true>), grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
-                                           b.counters + kMaxSegments + k, b.counters + 3 * kMaxSegments + k);
+                        if (brute)
+                            hipLaunchKernelGGL((k_shadow_brute<kRender, true>), grid_r, dim3(kTraceBlock), 0, s, c,
+                                               ntris, b.sq, cnt);
+                        else
+                            hipLaunchKernelGGL((k_shadow<kRender, true>), grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
+                                               cnt, b.counters + 3 * kMaxSegments + k);
                         return;
                     }
                 }
-                hipLaunchKernelGGL(k_shadow<kMode>, grid_t, dim3(kTraceBlock), 0, s, c, b.sq,
-                                   b.counters + kMaxSegments + k, b.counters + 3 * kMaxSegments + k);
+                if (brute)
+                    hipLaunchKernelGGL(k_shadow_brute<kMode>, grid_r, dim3(kTraceBlock), 0, s, c, ntris, b.sq, cnt);
+                else
+                    hipLaunchKernelGGL(k_shadow<kMode>, grid_t, dim3(kTraceBlock), 0, s, c, b.sq, cnt,
+                                       b.counters + 3 * kMaxSegments + k);
             });
             if (e2 != hipSuccess) return hip_fail(ctx, e2, "k_shadow");
             if (kMode != kRender) {
@@ -1389,8 +1422,12 @@ struct Pipeline {
             const Queue& qi = b.q[k & 1];
             const Queue& qo = b.q[(k + 1) & 1];
             e = timed(ctx, kKExtend, [&]() {
-                hipLaunchKernelGGL(k_extend, grid_t, dim3(kTraceBlock), 0, s, c.nodes, c.tris, qi, b.counters + k,
-                                   b.counters + 2 * kMaxSegments + k, b.hits);
+                if (brute)
+                    hipLaunchKernelGGL(k_extend_brute, grid_r, dim3(kTraceBlock), 0, s, c.tris, ntris, qi,
+                                       b.counters + k, b.hits);
+                else
+                    hipLaunchKernelGGL(k_extend, grid_t, dim3(kTraceBlock), 0, s, c.nodes, c.tris, qi, b.counters + k,
+                                       b.counters + 2 * kMaxSegments + k, b.hits);
             });
             if (e != hipSuccess) return hip_fail(ctx, e, "k_extend");
             e = timed(ctx, kKShade, [&]() {

@@ -532,6 +532,41 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
     }
 }
 
+// Scenes of at most kBruteMaxTris triangles skip the BVH: every lane tests every triangle
+// record, in the same order, so the records are read at wave-uniform addresses and no lane
+// idles in a divergent descent -- the loop of the reference's NaiveScene
+// (scene.naive.cpp:31-124) with the watertight test.  A box room of a few dozen triangles
+// costs a BVH descent through overlapping wall boxes that several times this loop costs.
+constexpr uint32_t kBruteMaxTris = 64;
+
+// One ray per thread over grid-strided chunks of the batch; `active` lanes past the end test
+// nothing.  Any hit stops once every active lane of the wave has a hit.
+template <bool kAny, class Source, class Sink>
+__device__ __forceinline__ void trace_brute(const TriRecord* __restrict__ tris, uint32_t ntris, uint32_t n,
+                                            Source& src, Sink& sink) {
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        const RayIn r = active ? src(i) : RayIn{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f};
+        const Shear sh = make_shear(r);
+        float tmax = r.tmax;
+        HitOut h{tmax, 0u, 0xffffffffu, 0.f, 0.f};
+        bool found = false;
+        for (uint32_t j = 0; j < ntris; j++) {
+            const float4* tp = reinterpret_cast<const float4*>(tris + j);
+            const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+            float t, U, V, W, det;
+            if (active && !(kAny && found) && watertight(sh, v0, v1, v2, r.tmin, tmax, t, U, V, W, det)) {
+                found = true;
+                tmax = t;
+                accept_hit<kAny>(h, t, j, __float_as_uint(v0.w), V, W, det);
+            }
+            if (kAny && __ballot(active && !found) == 0ull) break;
+        }
+        if (active) sink(i, found, h, r);
+    }
+}
+
 // Barycentrics of the recorded winner (weights of v1 and v2): the FP32 weights kept at
 // acceptance, or the winner re-evaluated in FP64.
 __device__ __forceinline__ void winner_barycentrics(const TriRecord* __restrict__ tris, const RayIn& r,
