@@ -81,6 +81,10 @@ __device__ __forceinline__ Shear make_shear(const RayIn& r) {
     return s;
 }
 
+__device__ __forceinline__ bool watertight_tail(float ax, float ay, float bx, float by, float cx, float cy, float za,
+                                                float zb, float zc, float tmin, float tmax, float& t, float& U,
+                                                float& V, float& W, float& det);
+
 // Watertight ray/triangle test.  Returns true with t strictly inside (tmin, tmax);
 // U, V, W are the scaled barycentric weights of v0, v1, v2 and det = U + V + W.
 __device__ __forceinline__ bool watertight(const Shear& s, float4 v0, float4 v1, float4 v2, float tmin, float tmax,
@@ -98,6 +102,53 @@ __device__ __forceinline__ bool watertight(const Shear& s, float4 v0, float4 v1,
     const float ax = Ax - s.Sx * Az, ay = Ay - s.Sy * Az;
     const float bx = Bx - s.Sx * Bz, by = By - s.Sy * Bz;
     const float cx = Cx - s.Sx * Cz, cy = Cy - s.Sy * Cz;
+    return watertight_tail(ax, ay, bx, by, cx, cy, s.Sz * Az, s.Sz * Bz, s.Sz * Cz, tmin, tmax, t, U, V, W, det);
+}
+
+// The shear of make_shear as three world-frame rows, for triangles whose vertices are the
+// same in every lane (trace_brute), where selecting a vertex's permuted components costs
+// more than the shear: with A = v - o, x' = A . (e_kx - Sx e_kz), y' = A . (e_ky - Sy e_kz)
+// and z = A . e_kz (exact), by fused multiply-adds.  Every triangle sees a vertex through
+// the same arithmetic, so shared edges stay closed.
+struct ShearRows {
+    float rx[3], ry[3], rz[3];
+    float o[3];
+    float Sz;
+};
+
+__device__ __forceinline__ ShearRows make_shear_rows(const RayIn& r, const Shear& s) {
+    ShearRows w;
+    for (int k = 0; k < 3; k++) {
+        w.rx[k] = (k == s.kx ? 1.f : 0.f) - (k == s.kz ? s.Sx : 0.f);
+        w.ry[k] = (k == s.ky ? 1.f : 0.f) - (k == s.kz ? s.Sy : 0.f);
+        w.rz[k] = k == s.kz ? 1.f : 0.f;
+    }
+    w.o[0] = r.ox;
+    w.o[1] = r.oy;
+    w.o[2] = r.oz;
+    w.Sz = s.Sz;
+    return w;
+}
+
+__device__ __forceinline__ bool watertight_rows(const ShearRows& s, float4 v0, float4 v1, float4 v2, float tmin,
+                                                float tmax, float& t, float& U, float& V, float& W, float& det) {
+    auto row = [](const float r[3], float x, float y, float z) { return fmaf(z, r[2], fmaf(y, r[1], x * r[0])); };
+    const float Ax = v0.x - s.o[0], Ay = v0.y - s.o[1], Az = v0.z - s.o[2];
+    const float Bx = v1.x - s.o[0], By = v1.y - s.o[1], Bz = v1.z - s.o[2];
+    const float Cx = v2.x - s.o[0], Cy = v2.y - s.o[1], Cz = v2.z - s.o[2];
+    const float ax = row(s.rx, Ax, Ay, Az), ay = row(s.ry, Ax, Ay, Az);
+    const float bx = row(s.rx, Bx, By, Bz), by = row(s.ry, Bx, By, Bz);
+    const float cx = row(s.rx, Cx, Cy, Cz), cy = row(s.ry, Cx, Cy, Cz);
+    const float za = row(s.rz, Ax, Ay, Az), zb = row(s.rz, Bx, By, Bz), zc = row(s.rz, Cx, Cy, Cz);
+    return watertight_tail(ax, ay, bx, by, cx, cy, s.Sz * za, s.Sz * zb, s.Sz * zc, tmin, tmax, t, U, V, W, det);
+}
+
+// Watertight test after the shear: (ax, ay) ... the sheared 2D vertices, za ... their
+// scaled distances along the ray.
+__device__ __forceinline__ bool watertight_tail(float ax, float ay, float bx, float by, float cx, float cy, float za,
+                                                float zb, float zc, float tmin, float tmax, float& t, float& U,
+                                                float& V, float& W, float& det) {
+#pragma clang fp contract(off)
     U = cx * by - cy * bx;
     V = ax * cy - ay * cx;
     W = bx * ay - by * ax;
@@ -110,7 +161,7 @@ __device__ __forceinline__ bool watertight(const Shear& s, float4 v0, float4 v1,
     if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
     det = U + V + W;
     if (det == 0.f) return false;
-    const float T = U * (s.Sz * Az) + V * (s.Sz * Bz) + W * (s.Sz * Cz);
+    const float T = U * za + V * zb + W * zc;
     t = T * __builtin_amdgcn_rcpf(det);  // within 3 ulp of T / det
     return t > tmin && t < tmax;
 }
@@ -548,7 +599,7 @@ __device__ __forceinline__ void trace_brute(const TriRecord* __restrict__ tris, 
         const uint32_t i = base + threadIdx.x;
         const bool active = i < n;
         const RayIn r = active ? src(i) : RayIn{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f};
-        const Shear sh = make_shear(r);
+        const ShearRows sh = make_shear_rows(r, make_shear(r));
         float tmax = r.tmax;
         HitOut h{tmax, 0u, 0xffffffffu, 0.f, 0.f};
         bool found = false;
@@ -556,7 +607,7 @@ __device__ __forceinline__ void trace_brute(const TriRecord* __restrict__ tris, 
             const float4* tp = reinterpret_cast<const float4*>(tris + j);
             const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
             float t, U, V, W, det;
-            if (active && !(kAny && found) && watertight(sh, v0, v1, v2, r.tmin, tmax, t, U, V, W, det)) {
+            if (active && !(kAny && found) && watertight_rows(sh, v0, v1, v2, r.tmin, tmax, t, U, V, W, det)) {
                 found = true;
                 tmax = t;
                 accept_hit<kAny>(h, t, j, __float_as_uint(v0.w), V, W, det);
