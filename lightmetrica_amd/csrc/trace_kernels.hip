@@ -74,6 +74,25 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_any(const Node4* __restri
     trace_persistent<true, false, true>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
+// Scenes of at most kBruteMaxTris triangles: every triangle per ray (trace_brute).
+template <bool kRefine>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest_brute(const TriRecord* __restrict__ tris, uint32_t ntris,
+                                                                     RaysSoA rays, uint32_t n, float* __restrict__ t_out,
+                                                                     float* __restrict__ b0_out, float* __restrict__ b1_out,
+                                                                     int32_t* __restrict__ prim_out) {
+    SoASource src{rays};
+    ClosestSink<kRefine> sink{tris, t_out, b0_out, b1_out, prim_out};
+    trace_brute<false>(tris, ntris, n, src, sink);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_any_brute(const TriRecord* __restrict__ tris, uint32_t ntris,
+                                                                 RaysSoA rays, uint32_t n,
+                                                                 uint8_t* __restrict__ occluded) {
+    SoASource src{rays};
+    OcclusionSink sink{occluded};
+    trace_brute<true>(tris, ntris, n, src, sink);
+}
+
 __global__ void k_zero_next(uint32_t* p) {
     if (threadIdx.x == 0) *p = 0u;
 }
@@ -116,10 +135,26 @@ template <class K> static uint32_t persistent_blocks(lmgpu_ctx* ctx, K kernel, u
     return uint32_t(std::max<uint64_t>(1, std::min(full, need)));
 }
 
+// Grid of the brute-force kernels: 16 blocks per CU at most, walking the batch.
+static uint32_t brute_blocks(lmgpu_ctx* ctx, uint32_t n) {
+    const uint64_t need = (uint64_t(n) + kTraceBlock - 1) / kTraceBlock;
+    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(need, 16ull * uint64_t(std::max(ctx->num_cus, 1)))));
+}
+
 hipError_t launch_trace_closest(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, float* t, float* b0, float* b1,
                                 int32_t* prim, bool refine) {
     uint32_t* next = ctx->next_counter;
     const dim3 block(kTraceBlock);
+    const uint32_t ntris = uint32_t(ctx->tris.count);
+    if (ntris <= kBruteMaxTris)
+        return timed(ctx, kKTraceClosest, [&]() {
+            if (refine)
+                hipLaunchKernelGGL(k_trace_closest_brute<true>, dim3(brute_blocks(ctx, n)), block, 0, ctx->stream,
+                                   ctx->tris.ptr, ntris, soa_of(rays), n, t, b0, b1, prim);
+            else
+                hipLaunchKernelGGL(k_trace_closest_brute<false>, dim3(brute_blocks(ctx, n)), block, 0, ctx->stream,
+                                   ctx->tris.ptr, ntris, soa_of(rays), n, t, b0, b1, prim);
+        });
     return timed(ctx, kKTraceClosest, [&]() {
         hipLaunchKernelGGL(k_zero_next, dim3(1), dim3(64), 0, ctx->stream, next);
         if (refine)
@@ -133,6 +168,12 @@ hipError_t launch_trace_closest(lmgpu_ctx* ctx, const float* const* rays, uint32
 
 hipError_t launch_trace_any(lmgpu_ctx* ctx, const float* const* rays, uint32_t n, uint8_t* occluded) {
     uint32_t* next = ctx->next_counter;
+    const uint32_t ntris = uint32_t(ctx->tris.count);
+    if (ntris <= kBruteMaxTris)
+        return timed(ctx, kKTraceAny, [&]() {
+            hipLaunchKernelGGL(k_trace_any_brute, dim3(brute_blocks(ctx, n)), dim3(kTraceBlock), 0, ctx->stream,
+                               ctx->tris.ptr, ntris, soa_of(rays), n, occluded);
+        });
     return timed(ctx, kKTraceAny, [&]() {
         hipLaunchKernelGGL(k_zero_next, dim3(1), dim3(64), 0, ctx->stream, next);
         hipLaunchKernelGGL(k_trace_any, dim3(persistent_blocks(ctx, k_trace_any, n)), dim3(kTraceBlock), 0,
