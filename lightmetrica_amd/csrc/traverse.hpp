@@ -588,7 +588,9 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
 // idles in a divergent descent -- the loop of the reference's NaiveScene
 // (scene.naive.cpp:31-124) with the watertight test.  A box room of a few dozen triangles
 // costs a BVH descent through overlapping wall boxes that several times this loop costs.
-constexpr uint32_t kBruteMaxTris = 64;
+// Measured crossover (the room with 0-16 extra boxes, 256x256x16 pt): 36 triangles 0.90 ->
+// 0.44 ms, 90: 1.15 -> 0.82, 138: 1.09 -> 1.06, 234: 1.15 -> 1.51.
+constexpr uint32_t kBruteMaxTris = 128;
 
 // One ray per thread over grid-strided chunks of the batch; `active` lanes past the end test
 // nothing.  Any hit stops once every active lane of the wave has a hit.

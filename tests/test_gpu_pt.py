@@ -94,13 +94,20 @@ def test_light_across_camera_plane_same_streams(gpu, oracle, renderer):
 
 
 def random_scene(case: int, res=32, spp=16):
-    """A random open scene: a ground quad, 1-6 rotated boxes with random diffuse / mirror /
-    diffuse-mirror materials, 1-3 area lights of random emission, a constant environment
-    in half of the cases, a random camera and random path-length settings."""
+    """A random open scene: a ground quad (in odd cases split into 10 x 10 quads, so the scene
+    has more than 128 triangles and is traced through its BVH rather than triangle by
+    triangle), 1-6 rotated boxes with random diffuse / mirror / diffuse-mirror materials, 1-3
+    area lights of random emission, a constant environment in half of the cases, a random
+    camera and random path-length settings."""
     rng = np.random.default_rng(1000 + case)
     b = scenes._Builder()
     entries = [(scenes.BSDF_DIFFUSE, tuple(rng.uniform(0.2, 0.9, 3)))]
-    b.quad((-2, 0, -2), (0, 0, 4), (4, 0, 0), 0)
+    if case % 2:
+        for gx in range(10):
+            for gz in range(10):
+                b.quad((-2 + 0.4 * gx, 0, -2 + 0.4 * gz), (0, 0, 0.4), (0.4, 0, 0), 0)
+    else:
+        b.quad((-2, 0, -2), (0, 0, 4), (4, 0, 0), 0)
     for _ in range(int(rng.integers(1, 7))):
         kind = int(rng.choice([scenes.BSDF_DIFFUSE, scenes.BSDF_MIRROR, scenes.BSDF_DIFFUSEMIRROR]))
         entries.append((kind, tuple(rng.uniform(0.1, 0.95, 3))))
