@@ -597,6 +597,44 @@ constexpr uint32_t kBruteMaxTris = 128;
 template <bool kAny, class Source, class Sink>
 __device__ __forceinline__ void trace_brute(const TriRecord* __restrict__ tris, uint32_t ntris, uint32_t n,
                                             Source& src, Sink& sink) {
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    if (n <= waves) {
+        // At most one ray per wave (a late bounce, a small batch): the lanes share the ray and
+        // split the triangles, so the ray costs one pass over ceil(ntris / 64) records instead
+        // of a lane's walk over all of them.  The nearest hit (the lowest lane among equals)
+        // or, for any hit, the lowest lane with a hit reports the ray.
+        const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (w >= n) return;  // wave-uniform
+        const uint32_t lane = __lane_id();
+        const RayIn r = src(w);
+        const ShearRows sh = make_shear_rows(r, make_shear(r));
+        float tmax = r.tmax;
+        HitOut h{tmax, 0u, 0xffffffffu, 0.f, 0.f};
+        bool found = false;
+        for (uint32_t j = lane; j < ntris; j += 64u) {
+            const float4* tp = reinterpret_cast<const float4*>(tris + j);
+            const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+            float t, U, V, W, det;
+            if (!(kAny && found) && watertight_rows(sh, v0, v1, v2, r.tmin, tmax, t, U, V, W, det)) {
+                found = true;
+                tmax = t;
+                accept_hit<kAny>(h, t, j, __float_as_uint(v0.w), V, W, det);
+            }
+        }
+        const unsigned long long hits = __ballot(found);
+        if (hits == 0ull) {
+            if (lane == 0u) sink(w, false, h, r);
+            return;
+        }
+        int winner = __ffsll(static_cast<long long>(hits)) - 1;
+        if (!kAny) {
+            float best = found ? h.t : __builtin_inff();
+            for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
+            winner = __ffsll(static_cast<long long>(__ballot(found && h.t == best))) - 1;
+        }
+        if (int(lane) == winner) sink(w, true, h, r);
+        return;
+    }
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < n;
