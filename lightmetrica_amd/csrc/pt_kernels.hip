@@ -77,6 +77,7 @@ struct PTConsts {
     size_t splat_vals;
     uint32_t splat_x0, splat_y0, splat_w, splat_h;  // the film rectangle the replicas cover
     const uint32_t* order;  // pt: batch sample of each camera-stage thread (screen-tile order), or nullptr
+    uint32_t ntris;         // triangle records (the shading kernels trace their own rays up to kBruteMaxTris)
     uint64_t seed;
     uint64_t base;  // global index of the batch's first sample
     uint32_t batch;  // samples in this batch
@@ -735,16 +736,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(const Node4* __restrict_
     trace_persistent<false, false, false>(nodes, tris, n, next, sm, src, sink, cnt);
 }
 
-// Extension rays of a scene of at most kBruteMaxTris triangles: every triangle per ray.
-__global__ __launch_bounds__(kTraceBlock) void k_extend_brute(const TriRecord* __restrict__ tris, uint32_t ntris,
-                                                              Queue q, const uint32_t* __restrict__ count,
-                                                              float4* __restrict__ hits) {
-    const uint32_t n = *count;
-    QueueSource src{q};
-    HitSink sink{tris, hits};
-    trace_brute<false>(tris, ntris, n, src, sink);
-}
-
 // dL/dR of the vertex's material: Q / R per channel (the contributions still in Q all carry
 // R once).
 __device__ __forceinline__ void grad_material(GradCache& g, const PTConsts& c, uint32_t sid, uint32_t mat, V3 R) {
@@ -758,7 +749,7 @@ __device__ __forceinline__ void grad_material(GradCache& g, const PTConsts& c, u
 // over a device-filling grid (16 blocks per CU; 4 for the gradient pass, so that each
 // block's gradient table collects many paths before it is flushed): a late segment's
 // nearly empty queue then costs a launch of that grid, not of one block per queue slot.
-template <int kMode>
+template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                        const float4* __restrict__ hits, Queue out,
                                                        uint32_t* out_count) {
@@ -786,7 +777,11 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
     float pdf = 0.f;
     SampleStream rng;
     rng.init(c.seed, c.base + meta.x, meta.z);
-    if (valid && surface_at(c, o, d, ro.w, hits[j], s)) {
+    // kFuse (scenes of at most kBruteMaxTris triangles): the segment's ray is traced here
+    // instead of by a separate extension launch.
+    const float4 hit = kFuse ? brute_closest(c.tris, c.ntris, RayIn{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, ro.w, kInf}, valid)
+                             : hits[j];
+    if (valid && surface_at(c, o, d, ro.w, hit, s)) {
         if (s.light >= 0 && dot(s.f.sn, -d) > 0.f) {
             emits = true;
             Le = light_Le(c, s.light);
@@ -829,7 +824,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
 // pt.direct / pt.mis: the hit half of one loop iteration (pathtrace.direct.cpp:304-336,
 // pathtrace.mis.cpp:318-367) followed by the next iteration up to the next ray
 // (NEE :197-248 / :194-262, RR, BSDF sampling).
-template <int kMode>
+template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                         const float4* __restrict__ hits, Queue out,
                                                         uint32_t* out_count, ShadowQueue sq, uint32_t* shadow_count) {
@@ -852,7 +847,9 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     Surf s;
     s.mat = kBlackMat;
     s.light = -1;
-    bool alive = valid && surface_at(c, o, d, ro.w, hits[j], s);
+    const float4 hit = kFuse ? brute_closest(c.tris, c.ntris, RayIn{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, ro.w, kInf}, valid)
+                             : hits[j];
+    bool alive = valid && surface_at(c, o, d, ro.w, hit, s);
     bool emits = false;
     V3 Le_w = v3s(0.f);
     if (alive && s.light >= 0) {
@@ -1330,6 +1327,7 @@ struct Pipeline {
         segments = p->max_path_vertices - 1;
         is_pt = p->renderer == LMGPU_PT;
         brute = ctx->tris.count <= kBruteMaxTris;
+        c.ntris = uint32_t(ctx->tris.count);
         // A BVH larger than the caches: camera rays in screen-tile order (k_tile_keys), whose
         // raster numbers follow key_skip others in each stream.
         c.order = nullptr;
@@ -1421,19 +1419,25 @@ struct Pipeline {
         for (int k = 0; k < segments; k++) {
             const Queue& qi = b.q[k & 1];
             const Queue& qo = b.q[(k + 1) & 1];
-            e = timed(ctx, kKExtend, [&]() {
-                if (brute)
-                    hipLaunchKernelGGL(k_extend_brute, grid_r, dim3(kTraceBlock), 0, s, c.tris, ntris, qi,
-                                       b.counters + k, b.hits);
-                else
+            // Scenes of at most kBruteMaxTris triangles: the shading kernel traces the segment's
+            // rays itself (kFuse), so there is no extension launch and no hit record.
+            if (!brute) {
+                e = timed(ctx, kKExtend, [&]() {
                     hipLaunchKernelGGL(k_extend, grid_t, dim3(kTraceBlock), 0, s, c.nodes, c.tris, qi, b.counters + k,
                                        b.counters + 2 * kMaxSegments + k, b.hits);
-            });
-            if (e != hipSuccess) return hip_fail(ctx, e, "k_extend");
+                });
+                if (e != hipSuccess) return hip_fail(ctx, e, "k_extend");
+            }
             e = timed(ctx, kKShade, [&]() {
-                if (is_pt)
+                if (is_pt && brute)
+                    hipLaunchKernelGGL((k_shade_pt<kMode, true>), grid_s, dim3(kPTBlock), 0, s, c, qi, b.counters + k,
+                                       b.hits, qo, b.counters + k + 1);
+                else if (is_pt)
                     hipLaunchKernelGGL(k_shade_pt<kMode>, grid_s, dim3(kPTBlock), 0, s, c, qi, b.counters + k, b.hits,
                                        qo, b.counters + k + 1);
+                else if (brute)
+                    hipLaunchKernelGGL((k_shade_nee<kMode, true>), grid_s, dim3(kPTBlock), 0, s, c, qi, b.counters + k,
+                                       b.hits, qo, b.counters + k + 1, b.sq, b.counters + kMaxSegments + k + 1);
                 else
                     hipLaunchKernelGGL(k_shade_nee<kMode>, grid_s, dim3(kPTBlock), 0, s, c, qi, b.counters + k,
                                        b.hits, qo, b.counters + k + 1, b.sq, b.counters + kMaxSegments + k + 1);

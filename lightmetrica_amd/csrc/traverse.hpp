@@ -658,6 +658,29 @@ __device__ __forceinline__ void trace_brute(const TriRecord* __restrict__ tris, 
     }
 }
 
+// The closest hit of one ray over every triangle record, as the extension stage stores it:
+// (t, b0, b1, prim bits), prim = 0xffffffff for a miss.  The lane-per-ray loop of
+// trace_brute, for the shading kernels that trace their own rays in scenes of at most
+// kBruteMaxTris triangles; every lane of the wave runs the loop, inactive lanes test nothing.
+__device__ __forceinline__ float4 brute_closest(const TriRecord* __restrict__ tris, uint32_t ntris, const RayIn& r,
+                                                bool active) {
+    const ShearRows sh = make_shear_rows(r, make_shear(r));
+    float tmax = r.tmax;
+    HitOut h{tmax, 0u, 0xffffffffu, 0.f, 0.f};
+    bool found = false;
+    for (uint32_t j = 0; j < ntris; j++) {
+        const float4* tp = reinterpret_cast<const float4*>(tris + j);
+        const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+        float t, U, V, W, det;
+        if (active && watertight_rows(sh, v0, v1, v2, r.tmin, tmax, t, U, V, W, det)) {
+            found = true;
+            tmax = t;
+            accept_hit<false>(h, t, j, __float_as_uint(v0.w), V, W, det);
+        }
+    }
+    return found ? make_float4(h.t, h.b0, h.b1, __uint_as_float(h.prim)) : make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
+}
+
 // Barycentrics of the recorded winner (weights of v1 and v2): the FP32 weights kept at
 // acceptance, or the winner re-evaluated in FP64.
 __device__ __forceinline__ void winner_barycentrics(const TriRecord* __restrict__ tris, const RayIn& r,
