@@ -595,6 +595,23 @@ __device__ __forceinline__ void put_shadow(const ShadowQueue& sq, uint32_t slot,
     sq.sid[slot] = sid;
 }
 
+// Render of a scene of at most kBruteMaxTris triangles: the connection's shadow ray is traced
+// at once (any hit over every triangle record) and its unoccluded contribution added, as
+// ShadowSink does for a queued ray; kSplat: the camera vertex's connections (splat_add).
+// Every lane calls it: the triangle loop is wave-uniform.
+template <bool kSplat>
+__device__ __forceinline__ void shadow_now(const PTConsts& c, const ShadowRec& rec, uint32_t sid) {
+    const RayIn r{rec.o.x, rec.o.y, rec.o.z, rec.d.x, rec.d.y, rec.d.z, kEps, rec.o.w};
+    const bool occluded = brute_any(c.tris, c.ntris, r, rec.emit);
+    if (!rec.emit || occluded) return;
+    const V3 Le = light_Le(c, __float_as_int(rec.c.w));
+    const uint32_t pixel = __float_as_uint(rec.d.w);
+    const V3 x = v3(rec.c.x, rec.c.y, rec.c.z) * Le;
+    if (kSplat) splat_add(c, pixel, x);
+    else film_add(c, pixel, x);
+    if (c.record) record_add(c, sid, pixel, x);
+}
+
 // ---------------------------------------------------------------------------------
 // Kernels
 
@@ -640,7 +657,9 @@ __global__ void k_tile_keys(PTConsts c, int skip, uint32_t* __restrict__ keys, u
 
 // Camera vertex.  pt: pathtrace.cpp:167-192.  pt.direct / pt.mis: the first loop
 // iteration of pathtrace.direct.cpp:172-303 (NEE from the camera, RR, camera sampling).
-template <int kMode>
+// kFuse (render of a scene of at most kBruteMaxTris triangles): the camera vertex's
+// connection is shadow-tested here (shadow_now) instead of queued.
+template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_t* live_count, ShadowQueue sq,
                                                   uint32_t* shadow_count) {
     const uint32_t slot0 = blockIdx.x * kPTBlock + threadIdx.x;
@@ -674,8 +693,13 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
     v.camera = true;
     v.wi = v3s(0.f);
     const ShadowRec rec = nee(c, v, T, 1, 0u, rng, mis, alive, kMode != kRender);
-    const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
-    if (rec.emit) put_shadow(sq, sslot, rec, i);
+    if constexpr (kMode == kRender && kFuse) {
+        (void)block_append<0>(shadow_count, rec.emit);  // counts the shadow rays traced
+        shadow_now<true>(c, rec, i);
+    } else {
+        const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
+        if (rec.emit) put_shadow(sq, sslot, rec, i);
+    }
     if (alive && c.rr_depth != -1 && 1 >= c.rr_depth) {
         const float q = fminf(0.5f, luminance(T));
         if (rng.next() > q) alive = false;
@@ -889,8 +913,13 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     v.wi = -d;
     v.m = alive ? mat_info(c, s.mat) : MatInfo{LMGPU_BSDF_MIRROR, v3s(0.f)};
     const ShadowRec rec = nee(c, v, T, n, meta.w, rng, mis, alive && mat_nondelta(v.m), kMode != kRender);
-    const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
-    if (rec.emit) put_shadow(sq, sslot, rec, meta.x);
+    if constexpr (kMode == kRender && kFuse) {
+        (void)block_append<0>(shadow_count, rec.emit);  // counts the shadow rays traced
+        shadow_now<false>(c, rec, meta.x);
+    } else {
+        const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
+        if (rec.emit) put_shadow(sq, sslot, rec, meta.x);
+    }
     if (alive && c.rr_depth != -1 && n >= c.rr_depth) {
         const float q = fminf(0.5f, luminance(T));
         if (rng.next() > q) alive = false;
@@ -1372,14 +1401,21 @@ struct Pipeline {
         // Shade grids: 16 blocks per CU; the gradient pass 4, each flushing its gradient table once.
         const dim3 grid_s(std::min<uint32_t>(grid_c.x, (kMode == kAdjGrad ? 4u : 16u) * uint32_t(std::max(ctx->num_cus, 1))));
         hipError_t sort_e = hipSuccess;
+        // Render of a scene of at most kBruteMaxTris triangles: shadow rays are traced where
+        // they are made (shadow_now), so no shadow launches.
+        const bool fuse_shadow = brute && kMode == kRender;
         hipError_t e = timed(ctx, kKGen, [&]() {
             if (c.order) {
                 hipLaunchKernelGGL(k_tile_keys, grid_b, dim3(kPTBlock), 0, s, c, key_skip, keys_in, vals_in);
                 size_t tb = sort_temp_bytes;
                 sort_e = sort_tile_keys(sort_temp, tb, keys_in, keys_out, vals_in, vals_out, nb, key_bits, s);
             }
-            hipLaunchKernelGGL(k_gen<kMode>, grid_b, dim3(kPTBlock), 0, s, c, b.q[0], b.counters + 0, b.sq,
-                               b.counters + kMaxSegments + 0);
+            if (fuse_shadow)
+                hipLaunchKernelGGL((k_gen<kMode, true>), grid_b, dim3(kPTBlock), 0, s, c, b.q[0], b.counters + 0, b.sq,
+                                   b.counters + kMaxSegments + 0);
+            else
+                hipLaunchKernelGGL(k_gen<kMode>, grid_b, dim3(kPTBlock), 0, s, c, b.q[0], b.counters + 0, b.sq,
+                                   b.counters + kMaxSegments + 0);
         });
         if (sort_e != hipSuccess) return hip_fail(ctx, sort_e, "camera ray order (sort)");
         if (e != hipSuccess) return hip_fail(ctx, e, "k_gen");
@@ -1412,7 +1448,7 @@ struct Pipeline {
             }
             return LMGPU_OK;
         };
-        if (!is_pt) {
+        if (!is_pt && !fuse_shadow) {
             lmgpu_status st = shadow(0);
             if (st != LMGPU_OK) return st;
         }
@@ -1443,7 +1479,7 @@ struct Pipeline {
                                        b.hits, qo, b.counters + k + 1, b.sq, b.counters + kMaxSegments + k + 1);
             });
             if (e != hipSuccess) return hip_fail(ctx, e, "k_shade");
-            if (!is_pt) {
+            if (!is_pt && !fuse_shadow) {
                 lmgpu_status st = shadow(k + 1);
                 if (st != LMGPU_OK) return st;
             }

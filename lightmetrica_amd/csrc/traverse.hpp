@@ -681,6 +681,22 @@ __device__ __forceinline__ float4 brute_closest(const TriRecord* __restrict__ tr
     return found ? make_float4(h.t, h.b0, h.b1, __uint_as_float(h.prim)) : make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
 }
 
+// Whether any triangle record blocks the ray (any hit of trace_brute, lane per ray); every
+// lane of the wave runs the loop until every active lane has a hit.
+__device__ __forceinline__ bool brute_any(const TriRecord* __restrict__ tris, uint32_t ntris, const RayIn& r,
+                                          bool active) {
+    const ShearRows sh = make_shear_rows(r, make_shear(r));
+    bool found = false;
+    for (uint32_t j = 0; j < ntris; j++) {
+        const float4* tp = reinterpret_cast<const float4*>(tris + j);
+        const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+        float t, U, V, W, det;
+        if (active && !found && watertight_rows(sh, v0, v1, v2, r.tmin, r.tmax, t, U, V, W, det)) found = true;
+        if (__ballot(active && !found) == 0ull) break;
+    }
+    return found;
+}
+
 // Barycentrics of the recorded winner (weights of v1 and v2): the FP32 weights kept at
 // acceptance, or the winner re-evaluated in FP64.
 __device__ __forceinline__ void winner_barycentrics(const TriRecord* __restrict__ tris, const RayIn& r,
