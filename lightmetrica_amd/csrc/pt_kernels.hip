@@ -6,10 +6,17 @@
 //   extend  -- closest-hit traversal of the live queue        (Scene::Intersect, scene.cpp:67-72)
 //   shade   -- emission, NEE, Russian roulette, BSDF sampling (pathtrace.cpp:203-262,
 //              pathtrace.direct.cpp:188-336, pathtrace.mis.cpp:185-367); live paths are
-//              compacted into the next queue with one atomic per wavefront
+//              compacted into the next queue with one atomic per block
 //   shadow  -- any-hit traversal of the NEE queue, then the film update
 //              (RenderUtils::Visible, renderutils.cpp:58-70)
 //   resolve -- film sums * W*H/N into the caller's film        (sched.mt.cpp:250-256)
+//
+// Scene-dependent forms of the same stages: on a BVH of 4 MB or more the camera stage hands
+// out path slots in screen-tile order (k_tile_keys + a radix sort); in a scene of at most
+// kBruteMaxTris triangles the shade kernels trace their own segment's rays over every
+// triangle record (no extend launch), and the render shadow-tests its connections where it
+// makes them (no shadow launch).  The next-event renderers add the camera vertex's light
+// connections to film replicas (splat_add) that are folded into the film at the end.
 //
 // Each path sample owns a Philox4x32-10 stream keyed by (seed, global sample index) and
 // draws from it in exactly the order the reference estimator calls Sampler::Next(), so
