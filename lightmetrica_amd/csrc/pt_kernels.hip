@@ -607,9 +607,10 @@ __device__ __forceinline__ void put_shadow(const ShadowQueue& sq, uint32_t slot,
 // ShadowSink does for a queued ray; kSplat: the camera vertex's connections (splat_add).
 // Every lane calls it: the triangle loop is wave-uniform.
 template <bool kSplat>
-__device__ __forceinline__ void shadow_now(const PTConsts& c, const ShadowRec& rec, uint32_t sid) {
+__device__ __forceinline__ void shadow_now(const PTConsts& c, const TriRecord* tris, const ShadowRec& rec,
+                                           uint32_t sid) {
     const RayIn r{rec.o.x, rec.o.y, rec.o.z, rec.d.x, rec.d.y, rec.d.z, kEps, rec.o.w};
-    const bool occluded = brute_any(c.tris, c.ntris, r, rec.emit);
+    const bool occluded = brute_any(tris, c.ntris, r, rec.emit);
     if (!rec.emit || occluded) return;
     const V3 Le = light_Le(c, __float_as_int(rec.c.w));
     const uint32_t pixel = __float_as_uint(rec.d.w);
@@ -621,6 +622,16 @@ __device__ __forceinline__ void shadow_now(const PTConsts& c, const ShadowRec& r
 
 // ---------------------------------------------------------------------------------
 // Kernels
+
+// The triangle records of a scene of at most kBruteMaxTris triangles copied into the block's
+// LDS, for the kernels that test every record per ray (kFuse): the loops then read them at
+// LDS latency.  Every thread of the block must call it (it synchronises the block).
+__device__ __forceinline__ const TriRecord* stage_tris(const PTConsts& c, TriRecord* lds) {
+    const uint32_t n = min(c.ntris, kBruteMaxTris);
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) lds[k] = c.tris[k];
+    __syncthreads();
+    return lds;
+}
 
 __device__ __forceinline__ void store_path(const Queue& q, uint32_t slot, V3 o, float tmin, V3 d, float pdf, V3 T,
                                            V3 aux, uint32_t sid, uint32_t n_flags, uint32_t dim, uint32_t pixel) {
@@ -669,6 +680,8 @@ __global__ void k_tile_keys(PTConsts c, int skip, uint32_t* __restrict__ keys, u
 template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_t* live_count, ShadowQueue sq,
                                                   uint32_t* shadow_count) {
+    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTris : 1];
+    const TriRecord* tris = kFuse ? stage_tris(c, tri_lds) : c.tris;
     const uint32_t slot0 = blockIdx.x * kPTBlock + threadIdx.x;
     const bool valid = slot0 < c.batch;
     const uint32_t i = c.order && valid ? c.order[slot0] : slot0;  // the batch sample of this thread
@@ -702,7 +715,7 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
     const ShadowRec rec = nee(c, v, T, 1, 0u, rng, mis, alive, kMode != kRender);
     if constexpr (kMode == kRender && kFuse) {
         (void)block_append<0>(shadow_count, rec.emit);  // counts the shadow rays traced
-        shadow_now<true>(c, rec, i);
+        shadow_now<true>(c, tris, rec, i);
     } else {
         const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
         if (rec.emit) put_shadow(sq, sslot, rec, i);
@@ -784,6 +797,8 @@ template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                        const float4* __restrict__ hits, Queue out,
                                                        uint32_t* out_count) {
+    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTris : 1];
+    const TriRecord* tris = kFuse ? stage_tris(c, tri_lds) : c.tris;
     __shared__ GradCache gc;
     if (kMode == kAdjGrad) gc_init(gc);
     const uint32_t count = *in_count;
@@ -810,7 +825,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, con
     rng.init(c.seed, c.base + meta.x, meta.z);
     // kFuse (scenes of at most kBruteMaxTris triangles): the segment's ray is traced here
     // instead of by a separate extension launch.
-    const float4 hit = kFuse ? brute_closest(c.tris, c.ntris, RayIn{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, ro.w, kInf}, valid)
+    const float4 hit = kFuse ? brute_closest(tris, c.ntris, RayIn{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, ro.w, kInf}, valid)
                              : hits[j];
     if (valid && surface_at(c, o, d, ro.w, hit, s)) {
         if (s.light >= 0 && dot(s.f.sn, -d) > 0.f) {
@@ -859,6 +874,8 @@ template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                         const float4* __restrict__ hits, Queue out,
                                                         uint32_t* out_count, ShadowQueue sq, uint32_t* shadow_count) {
+    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTris : 1];
+    const TriRecord* tris = kFuse ? stage_tris(c, tri_lds) : c.tris;
     __shared__ GradCache gc;
     if (kMode == kAdjGrad) gc_init(gc);
     const uint32_t count = *in_count;
@@ -878,7 +895,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     Surf s;
     s.mat = kBlackMat;
     s.light = -1;
-    const float4 hit = kFuse ? brute_closest(c.tris, c.ntris, RayIn{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, ro.w, kInf}, valid)
+    const float4 hit = kFuse ? brute_closest(tris, c.ntris, RayIn{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, ro.w, kInf}, valid)
                              : hits[j];
     bool alive = valid && surface_at(c, o, d, ro.w, hit, s);
     bool emits = false;
@@ -922,7 +939,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     const ShadowRec rec = nee(c, v, T, n, meta.w, rng, mis, alive && mat_nondelta(v.m), kMode != kRender);
     if constexpr (kMode == kRender && kFuse) {
         (void)block_append<0>(shadow_count, rec.emit);  // counts the shadow rays traced
-        shadow_now<false>(c, rec, meta.x);
+        shadow_now<false>(c, tris, rec, meta.x);
     } else {
         const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
         if (rec.emit) put_shadow(sq, sslot, rec, meta.x);
