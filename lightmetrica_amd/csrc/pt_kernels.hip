@@ -605,13 +605,15 @@ __device__ __forceinline__ void put_shadow(const ShadowQueue& sq, uint32_t slot,
 // Render of a scene of at most kBruteMaxTris triangles: the connection's shadow ray is traced
 // at once (any hit over every triangle record) and its unoccluded contribution added, as
 // ShadowSink does for a queued ray; kSplat: the camera vertex's connections (splat_add).
-// Every lane calls it: the triangle loop is wave-uniform.
-template <bool kSplat>
-__device__ __forceinline__ void shadow_now(const PTConsts& c, const TriRecord* tris, const ShadowRec& rec,
-                                           uint32_t sid) {
+// Every lane calls shadow_visible: the triangle loop is wave-uniform.
+__device__ __forceinline__ bool shadow_visible(const PTConsts& c, const TriRecord* tris, const ShadowRec& rec) {
     const RayIn r{rec.o.x, rec.o.y, rec.o.z, rec.d.x, rec.d.y, rec.d.z, kEps, rec.o.w};
     const bool occluded = brute_any(tris, c.ntris, r, rec.emit);
-    if (!rec.emit || occluded) return;
+    return rec.emit && !occluded;
+}
+
+template <bool kSplat>
+__device__ __forceinline__ void add_connection(const PTConsts& c, const ShadowRec& rec, uint32_t sid) {
     const V3 Le = light_Le(c, __float_as_int(rec.c.w));
     const uint32_t pixel = __float_as_uint(rec.d.w);
     const V3 x = v3(rec.c.x, rec.c.y, rec.c.z) * Le;
@@ -676,7 +678,8 @@ __global__ void k_tile_keys(PTConsts c, int skip, uint32_t* __restrict__ keys, u
 // Camera vertex.  pt: pathtrace.cpp:167-192.  pt.direct / pt.mis: the first loop
 // iteration of pathtrace.direct.cpp:172-303 (NEE from the camera, RR, camera sampling).
 // kFuse (render of a scene of at most kBruteMaxTris triangles): the camera vertex's
-// connection is shadow-tested here (shadow_now) instead of queued.
+// connection is shadow-tested here (shadow_visible) instead of queued, and added after the
+// sample's record is set up.
 template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_t* live_count, ShadowQueue sq,
                                                   uint32_t* shadow_count) {
@@ -713,9 +716,10 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
     v.camera = true;
     v.wi = v3s(0.f);
     const ShadowRec rec = nee(c, v, T, 1, 0u, rng, mis, alive, kMode != kRender);
+    bool lit = false;  // kFuse: the connection is unoccluded; added once the record is set up
     if constexpr (kMode == kRender && kFuse) {
         (void)block_append<0>(shadow_count, rec.emit);  // counts the shadow rays traced
-        shadow_now<true>(c, tris, rec, i);
+        lit = shadow_visible(c, tris, rec);
     } else {
         const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
         if (rec.emit) put_shadow(sq, sslot, rec, i);
@@ -743,6 +747,7 @@ __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_
         c.record[2ull * i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(alive ? pixel : kNoPixel));
         c.record[2ull * i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kNoPixel));
     }
+    if (kMode == kRender && kFuse && lit) add_connection<true>(c, rec, i);
 }
 
 // Extension rays: closest hit of every live path (persistent, refilling lanes).
@@ -939,7 +944,7 @@ __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, co
     const ShadowRec rec = nee(c, v, T, n, meta.w, rng, mis, alive && mat_nondelta(v.m), kMode != kRender);
     if constexpr (kMode == kRender && kFuse) {
         (void)block_append<0>(shadow_count, rec.emit);  // counts the shadow rays traced
-        shadow_now<false>(c, tris, rec, meta.x);
+        if (shadow_visible(c, tris, rec)) add_connection<false>(c, rec, meta.x);
     } else {
         const uint32_t sslot = block_append<0>(shadow_count, rec.emit);
         if (rec.emit) put_shadow(sq, sslot, rec, meta.x);
@@ -1426,7 +1431,7 @@ struct Pipeline {
         const dim3 grid_s(std::min<uint32_t>(grid_c.x, (kMode == kAdjGrad ? 4u : 16u) * uint32_t(std::max(ctx->num_cus, 1))));
         hipError_t sort_e = hipSuccess;
         // Render of a scene of at most kBruteMaxTris triangles: shadow rays are traced where
-        // they are made (shadow_now), so no shadow launches.
+        // they are made (shadow_visible, add_connection), so no shadow launches.
         const bool fuse_shadow = brute && kMode == kRender;
         hipError_t e = timed(ctx, kKGen, [&]() {
             if (c.order) {

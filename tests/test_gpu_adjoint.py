@@ -230,16 +230,22 @@ def test_autograd_function(gpu, renderer):
         assert losses[-1] < 0.8 * losses[0]
 
 
+@pytest.mark.parametrize("scene", ["boxes", "room"])
 @pytest.mark.parametrize("renderer", RENDERERS)
-def test_sample_record_backward_equals_two_replays(gpu, renderer):
+def test_sample_record_backward_equals_two_replays(gpu, renderer, scene):
     """The forward's sample record (pt: radiance and pixel per sample; pt.direct / pt.mis: the
     sums at the path's pixel and at the camera connection's pixel): its per-pixel sums make the
-    film, and a backward from it (one replay) gives the gradients of two replays."""
+    film, and a backward from it (one replay) gives the gradients of two replays.  "room" (36
+    triangles) runs the kernels that trace without the BVH and shadow-test where they make a
+    connection; "boxes" (7k triangles) the BVH and the shadow queue."""
     import torch
 
     from lightmetrica_amd.gpu import record_float4s
 
-    sc = scenes.gradient_scene(48, 48, 8, num_boxes=150, max_path_vertices=9, rr_depth=3)
+    if scene == "room":
+        sc = scenes.room_scene(48, 48, 8)
+    else:
+        sc = scenes.gradient_scene(48, 48, 8, num_boxes=150, max_path_vertices=9, rr_depth=3)
     gpu.upload(sc)
     gpu.build_bvh()
     n = sc.render.num_samples
