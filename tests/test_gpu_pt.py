@@ -221,6 +221,26 @@ def test_sharded_render_equals_whole(gpu):
     np.testing.assert_allclose(comb, whole, rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("renderer", ["pt", "pt.mis"])
+def test_tile_ordered_batches_and_shards_equal_whole(gpu, terrain, renderer):
+    """The 2M-triangle terrain (its BVH is large enough that the camera stage hands out path
+    slots in screen-tile order): batches narrower than the render, and two shards by
+    sample_offset, give the single-batch image up to summation order."""
+    sc, _ = terrain
+    gpu.upload(sc)
+    info = gpu.build_bvh()
+    assert info.device_bytes >= 4 << 20
+    n = sc.render.num_samples
+    whole, sw = gpu.render(renderer, film_precision="fp64")
+    narrow, sn = gpu.render(renderer, film_precision="fp64", wavefront_width=n // 3 + 7)
+    np.testing.assert_allclose(narrow, whole, rtol=1e-9, atol=1e-12)
+    assert sn.closest_rays == sw.closest_rays
+    h1, _ = gpu.render(renderer, film_precision="fp64", num_samples=n // 2)
+    h2, _ = gpu.render(renderer, film_precision="fp64", num_samples=n - n // 2, sample_offset=n // 2)
+    comb = (h1 * (n // 2) + h2 * (n - n // 2)) / n
+    np.testing.assert_allclose(comb, whole, rtol=1e-9, atol=1e-12)
+
+
 def test_furnace_closed_form(gpu):
     from tests.test_oracle_kat import furnace_scene
 
