@@ -84,7 +84,7 @@ struct PTConsts {
     size_t splat_vals;
     uint32_t splat_x0, splat_y0, splat_w, splat_h;  // the film rectangle the replicas cover
     const uint32_t* order;  // pt: batch sample of each camera-stage thread (screen-tile order), or nullptr
-    uint32_t ntris;         // triangle records (the shading kernels trace their own rays up to kBruteMaxTris)
+    uint32_t ntris;         // triangle records (the shading kernels trace their own rays up to kBruteMaxTrisPt)
     uint64_t seed;
     uint64_t base;  // global index of the batch's first sample
     uint32_t batch;  // samples in this batch
@@ -625,11 +625,11 @@ __device__ __forceinline__ void add_connection(const PTConsts& c, const ShadowRe
 // ---------------------------------------------------------------------------------
 // Kernels
 
-// The triangle records of a scene of at most kBruteMaxTris triangles copied into the block's
+// The triangle records of a scene of at most kBruteMaxTrisPt triangles copied into the block's
 // LDS, for the kernels that test every record per ray (kFuse): the loops then read them at
 // LDS latency.  Every thread of the block must call it (it synchronises the block).
 __device__ __forceinline__ const TriRecord* stage_tris(const PTConsts& c, TriRecord* lds) {
-    const uint32_t n = min(c.ntris, kBruteMaxTris);
+    const uint32_t n = min(c.ntris, kBruteMaxTrisPt);
     for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) lds[k] = c.tris[k];
     __syncthreads();
     return lds;
@@ -683,7 +683,7 @@ __global__ void k_tile_keys(PTConsts c, int skip, uint32_t* __restrict__ keys, u
 template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_gen(PTConsts c, Queue out, uint32_t* live_count, ShadowQueue sq,
                                                   uint32_t* shadow_count) {
-    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTris : 1];
+    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTrisPt : 1];
     const TriRecord* tris = kFuse ? stage_tris(c, tri_lds) : c.tris;
     const uint32_t slot0 = blockIdx.x * kPTBlock + threadIdx.x;
     const bool valid = slot0 < c.batch;
@@ -802,7 +802,7 @@ template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_shade_pt(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                        const float4* __restrict__ hits, Queue out,
                                                        uint32_t* out_count) {
-    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTris : 1];
+    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTrisPt : 1];
     const TriRecord* tris = kFuse ? stage_tris(c, tri_lds) : c.tris;
     __shared__ GradCache gc;
     if (kMode == kAdjGrad) gc_init(gc);
@@ -879,7 +879,7 @@ template <int kMode, bool kFuse = false>
 __global__ __launch_bounds__(kPTBlock) void k_shade_nee(PTConsts c, Queue in, const uint32_t* __restrict__ in_count,
                                                         const float4* __restrict__ hits, Queue out,
                                                         uint32_t* out_count, ShadowQueue sq, uint32_t* shadow_count) {
-    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTris : 1];
+    __shared__ TriRecord tri_lds[kFuse ? kBruteMaxTrisPt : 1];
     const TriRecord* tris = kFuse ? stage_tris(c, tri_lds) : c.tris;
     __shared__ GradCache gc;
     if (kMode == kAdjGrad) gc_init(gc);
@@ -1343,7 +1343,7 @@ struct Pipeline {
     uint32_t trace_grid;
     int segments;
     bool is_pt;
-    bool brute;  // at most kBruteMaxTris triangles: no BVH descent (trace_brute)
+    bool brute;  // at most kBruteMaxTrisPt (pt) / kBruteMaxTris triangles: no BVH descent
     // pt camera-stage order (large scenes): keys and samples in, sorted out, sort scratch.
     uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
     void* sort_temp = nullptr;
@@ -1384,7 +1384,7 @@ struct Pipeline {
         // Segments traced per path: at most max_path_vertices - 1 (pathtrace.cpp:196-199).
         segments = p->max_path_vertices - 1;
         is_pt = p->renderer == LMGPU_PT;
-        brute = ctx->tris.count <= kBruteMaxTris;
+        brute = ctx->tris.count <= (is_pt ? kBruteMaxTrisPt : kBruteMaxTris);
         c.ntris = uint32_t(ctx->tris.count);
         // A BVH larger than the caches: camera rays in screen-tile order (k_tile_keys), whose
         // raster numbers follow key_skip others in each stream.

@@ -588,9 +588,12 @@ __device__ __forceinline__ void trace_persistent(const Node4* __restrict__ nodes
 // idles in a divergent descent -- the loop of the reference's NaiveScene
 // (scene.naive.cpp:31-124) with the watertight test.  A box room of a few dozen triangles
 // costs a BVH descent through overlapping wall boxes that several times this loop costs.
-// Measured crossover (the room with 0-16 extra boxes, 256x256x16 pt): 36 triangles 0.90 ->
-// 0.44 ms, 90: 1.15 -> 0.82, 138: 1.09 -> 1.06, 234: 1.15 -> 1.51.
-constexpr uint32_t kBruteMaxTris = 128;
+// Measured crossovers (the room with extra boxes, 256x256x16, BVH -> every triangle): ray
+// batches as pt's extension stage, 138 triangles 1.09 -> 1.06 ms, 234: 1.15 -> 1.51; pt with
+// the rays traced inside its shading kernel, 138: 1.09 -> 0.81, 234: 1.15 -> 1.21; pt.direct
+// (a closest and an any hit per vertex), 36: 1.47 -> 0.73, 138: 1.92 -> 2.03.
+constexpr uint32_t kBruteMaxTris = 128;    // ray batches, pt.direct / pt.mis
+constexpr uint32_t kBruteMaxTrisPt = 192;  // pt (and the LDS copy of the records)
 
 // One ray per thread over grid-strided chunks of the batch; `active` lanes past the end test
 // nothing.  Any hit stops once every active lane of the wave has a hit.

@@ -150,6 +150,22 @@ def test_random_scenes_same_streams(gpu, oracle, case):
                                   f"max_pv {sc.render.max_path_vertices}, rr {sc.render.rr_depth})")
 
 
+@pytest.mark.parametrize("num_boxes", [2, 3])
+def test_brute_force_bounds_same_streams(gpu, oracle, num_boxes):
+    """The room with 2 or 3 extra boxes (138 / 186 triangles): pt traces them without the BVH
+    (its bound is 192 triangles), pt.direct / pt.mis through the BVH (bound 128); every
+    renderer draws the oracle's streams to its image."""
+    sc = scenes.gradient_scene(32, 32, 16, num_boxes=num_boxes, max_path_vertices=7, rr_depth=3)
+    assert 128 < sc.num_triangles <= 192
+    gpu.upload(sc)
+    gpu.build_bvh()
+    o = oracle.Oracle(sc, fp64=True, accel="bvh")
+    for renderer in RENDERERS:
+        g, _ = gpu.render(renderer, film_precision="fp64")
+        ref, _ = o.render(renderer)
+        same_stream_check(g, ref, f"room+{num_boxes} boxes/{renderer} ({sc.num_triangles} tris)")
+
+
 def test_fp32_film_matches_fp64_film(gpu):
     sc = scenes.room_scene(64, 64, 16)
     gpu.upload(sc)
